@@ -1,0 +1,227 @@
+"""beep_amd -- MI355X-native subgraph enumeration (BEEP's path) over libbeep.so.
+
+The Python layer mirrors the reference's in-process interface
+``graph::SG_Match<uint>(task, by, dev, cutoff, ndev).run(dataGraph, patGraph)``
+(subgraph_matching.cuh:113, 178) on top of the C ABI in include/beep.h.  All
+compute runs in libbeep.so (host C++ preparation + HIP kernels); there is no
+Python or CPU fallback for the enumeration.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import BeepError, check, lib
+
+SGM, SGC = 0, 1
+EMIT_NONE, EMIT_HASH, EMIT_BUFFER = 0, 1, 2
+
+__all__ = ["Graph", "Plan", "Prepared", "SG_Match", "SGM", "SGC", "BeepError",
+           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "tuple_hash", "mode_of"]
+
+
+def mode_of(task):
+    """Maps the reference's -m task names (Config.h:80-84) to a mode."""
+    if task in (SGM, SGC):
+        return task
+    if task == "sgm":
+        return SGM
+    if task == "sgc":
+        return SGC
+    raise ValueError(f"Unrecognized -mt option (Main TASK): {task}")
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(_ffi.u32p)
+
+
+class Graph:
+    """Unpeeled CSR with the reference's loader rules and its cutoff statistic."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        d = _ffi.GraphDesc()
+        check(lib().beep_graph_describe(self._h, C.byref(d)))
+        self.n, self.m = d.n, d.m
+        self.max_degree, self.cutoff = d.max_degree, d.cutoff
+        self.mean, self.sd = d.mean, d.sd
+
+    @classmethod
+    def load(cls, path, sort_edges=False):
+        h = C.c_void_p()
+        check(lib().beep_graph_load(str(path).encode(), 1 if sort_edges else 0, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, src, dst):
+        s, sp = _u32(src)
+        d, dp = _u32(dst)
+        if len(s) != len(d):
+            raise ValueError("src and dst differ in length")
+        h = C.c_void_p()
+        check(lib().beep_graph_from_edges(sp, dp, len(s), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale, samples, a=0.57, b=0.19, c=0.19, seed=1, n_keep=0):
+        h = C.c_void_p()
+        check(lib().beep_graph_rmat(scale, samples, a, b, c, seed, n_keep, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def chung_lu(cls, n, samples, gamma, i0, seed):
+        h = C.c_void_p()
+        check(lib().beep_graph_chung_lu(n, samples, gamma, i0, seed, C.byref(h)))
+        return cls(h.value)
+
+    def csr(self):
+        d = _ffi.GraphDesc()
+        check(lib().beep_graph_describe(self._h, C.byref(d)))
+        rp = np.ctypeslib.as_array(d.rowptr, shape=(d.n + 1,)).copy()
+        col = np.ctypeslib.as_array(d.col, shape=(max(d.m, 1),))[:d.m].copy()
+        return rp, col
+
+    def write_bel(self, path):
+        check(lib().beep_graph_write_bel(self._h, str(path).encode()))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().beep_graph_free(h)
+            self._h = C.c_void_p()
+
+
+class Plan:
+    """Query plan (matching order, back edges, symmetry chain, LUNMAT, asc/desc)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        d = _ffi.PlanDesc()
+        check(lib().beep_plan_describe(self._h, C.byref(d)))
+        k = d.k
+        self.k, self.mode, self.lunmat, self.asc = k, d.mode, d.lunmat, d.asc
+        self.max_qdeg, self.min_qdeg = d.max_qdeg, d.min_qdeg
+        self.seq = list(d.seq[:k])
+        self.qdeg = list(d.qdeg[:k])
+        self.qedge_ptr = list(d.qedge_ptr[:k + 1])
+        self.sym_ptr = list(d.sym_ptr[:k + 1])
+        self.qedge = [list(d.qedge[d.qedge_ptr[i]:d.qedge_ptr[i + 1]]) for i in range(k)]
+        self.sym = [list(d.sym[d.sym_ptr[i]:d.sym_ptr[i + 1]]) for i in range(k)]
+        self.width = k - self.lunmat
+
+    @classmethod
+    def from_mtx(cls, path, mode):
+        h = C.c_void_p()
+        check(lib().beep_plan_from_mtx(str(path).encode(), mode_of(mode), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, src, dst, mode):
+        s, sp = _u32(src)
+        d, dp = _u32(dst)
+        h = C.c_void_p()
+        check(lib().beep_plan_from_edges(sp, dp, len(s), mode_of(mode), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_undirected(cls, edges, mode):
+        """Pattern from undirected edges; both directions are listed, rows grouped."""
+        e = sorted({(a, b) for a, b in edges} | {(b, a) for a, b in edges})
+        return cls.from_edges([a for a, _ in e], [b for _, b in e], mode)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().beep_plan_free(h)
+            self._h = C.c_void_p()
+
+
+class Prepared:
+    """Peeled + degree-oriented graph with the root queue, resident in HBM once uploaded."""
+
+    def __init__(self, graph, plan):
+        h = C.c_void_p()
+        check(lib().beep_prepare(graph._h, plan._h, C.byref(h)))
+        self._h = h
+        self.graph, self.plan = graph, plan
+        d = self.describe()
+        self.n, self.m, self.n_roots = d.n, d.m, d.n_roots
+        self.alg_bytes = d.alg_bytes
+
+    def describe(self):
+        d = _ffi.PrepDesc()
+        check(lib().beep_prep_describe(self._h, C.byref(d)))
+        return d
+
+    def arrays(self):
+        d = self.describe()
+        A = np.ctypeslib.as_array
+        n, m = d.n, d.m
+        return dict(rowptr=A(d.rowptr, shape=(n + 1,)).copy(),
+                    col=A(d.col, shape=(max(m, 1),))[:m].copy(),
+                    ocol=A(d.ocol, shape=(max(m, 1),))[:m].copy(),
+                    nlow=A(d.nlow, shape=(max(n, 1),))[:n].copy(),
+                    roots=A(d.roots, shape=(max(d.n_roots, 1),))[:d.n_roots].copy(),
+                    q0_size=d.q0_size, boundary=d.boundary, cutoff=d.cutoff,
+                    max_deg=d.max_degree, peel_rounds=d.peel_rounds, asc=d.asc,
+                    n_hub_roots=d.n_hub_roots, n_lds_batches=d.n_lds_batches,
+                    alg_bytes=d.alg_bytes, max_root_degree=d.max_root_degree)
+
+    def upload(self, device=0):
+        check(lib().beep_prep_upload(self._h, device))
+
+    def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0):
+        o = _ffi.Opts()
+        lib().beep_opts_default(C.byref(o))
+        o.device, o.emit, o.lanes = device, emit, lanes
+        buf = None
+        if emit == EMIT_BUFFER and match_cap:
+            buf = np.zeros(match_cap * self.plan.width, dtype=np.uint32)
+            o.match_cap = match_cap
+            o.match_buf = buf.ctypes.data_as(_ffi.u32p)
+        r = _ffi.Result()
+        check(lib().beep_count(self._h, self.plan._h, C.byref(o), C.byref(r)))
+        out = dict(count=r.count, hash_sum=r.hash_sum, hash_xor=r.hash_xor,
+                   emitted=r.n_emitted, written=r.n_written, t_enum_ms=r.t_enum_ms,
+                   t_hub_ms=r.t_hub_ms, t_lds_ms=r.t_lds_ms, donations=r.donations)
+        if buf is not None:
+            out["matches"] = buf[:r.n_written * self.plan.width].reshape(-1, self.plan.width)
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().beep_prep_free(h)
+            self._h = C.c_void_p()
+
+
+class SG_Match:
+    """Reference-shaped driver: SG_Match(task, by, dev, cutoff, ndev).run(data, pattern).
+
+    ``cutoff`` is accepted for signature parity; like the reference CLI
+    (main.cu:132) the cutoff actually used is the one computed from the data
+    graph.  ``ndev`` must be 1: this build drives one GPU.
+    """
+
+    def __init__(self, task="sgm", by="node", dev=0, cutoff=768, ndev=1):
+        if ndev != 1:
+            raise ValueError("this build drives exactly one GPU (ndev must be 1)")
+        if by not in ("node", 0):
+            raise ValueError("only process-by-node is supported (the reference's only path)")
+        self.mode = mode_of(task)
+        self.dev = dev
+        self.last = None
+
+    def run(self, data, pattern, emit=EMIT_NONE):
+        plan = pattern if isinstance(pattern, Plan) else Plan.from_mtx(pattern, self.mode)
+        graph = data if isinstance(data, Graph) else Graph.load(data)
+        prep = Prepared(graph, plan)
+        prep.upload(self.dev)
+        self.last = prep.count(device=self.dev, emit=emit)
+        return self.last["count"]
+
+
+def tuple_hash(t):
+    a, ap = _u32(t)
+    return lib().beep_tuple_hash(ap, len(a))

@@ -1,0 +1,272 @@
+// capi.cpp -- the extern "C" boundary declared in include/beep.h.
+//
+// Every entry point converts exceptions into an errno-style return code and a
+// per-thread message (the reference exits the process on any error,
+// utils.cuh:27-40, Config.h:86-87).
+
+#include <sched.h>
+
+#include <algorithm>
+
+#include <cstdlib>
+#include <string>
+
+#include "internal.hpp"
+
+struct beep_graph { std::unique_ptr<beep::Graph> g; };
+struct beep_plan { std::unique_ptr<beep::Plan> p; };
+struct beep_prep { std::unique_ptr<beep::Prep> pp; };
+
+namespace beep {
+
+void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res);
+void upload_prep(Prep &pp, int device);
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+unsigned host_threads()
+{
+    static unsigned cached = 0;
+    if (cached) return cached;
+    unsigned t = 0;
+    if (const char *e = getenv("BEEP_THREADS")) t = (unsigned)atoi(e);
+    if (!t) {
+        if (const char *e = getenv("OMP_NUM_THREADS")) t = (unsigned)atoi(e);
+    }
+    if (!t) {
+        cpu_set_t cs;
+        CPU_ZERO(&cs);
+        if (sched_getaffinity(0, sizeof(cs), &cs) == 0) t = (unsigned)CPU_COUNT(&cs);
+    }
+    if (!t) t = std::thread::hardware_concurrency();
+    if (t > 64) t = 64;
+    cached = t ? t : 1;
+    return cached;
+}
+
+}  // namespace beep
+
+using beep::Error;
+
+template <class F>
+static int guarded(F &&f)
+{
+    try {
+        f();
+        return BEEP_OK;
+    } catch (const Error &e) {
+        beep::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        beep::set_last_error("out of host memory");
+        return BEEP_ENOMEM;
+    } catch (const std::exception &e) {
+        beep::set_last_error(e.what());
+        return BEEP_EINVAL;
+    }
+}
+
+extern "C" {
+
+const char *beep_last_error(void) { return beep::g_last_error.c_str(); }
+const char *beep_version(void) { return "beep-mi355x 0.1"; }
+
+int beep_graph_load(const char *path, int sort_edges, beep_graph **out)
+{
+    return guarded([&] {
+        if (!path || !out) throw Error(BEEP_EINVAL, "null argument");
+        std::vector<uint32_t> s, d;
+        beep::read_edges(path, s, d);
+        if (sort_edges) {
+            std::vector<uint64_t> k(s.size());
+            for (size_t i = 0; i < s.size(); i++) k[i] = ((uint64_t)s[i] << 32) | d[i];
+            std::sort(k.begin(), k.end());
+            for (size_t i = 0; i < s.size(); i++) {
+                s[i] = (uint32_t)(k[i] >> 32);
+                d[i] = (uint32_t)k[i];
+            }
+        }
+        auto g = beep::csr_from_edges(s.data(), d.data(), s.size());
+        *out = new beep_graph{std::move(g)};
+    });
+}
+
+int beep_graph_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t n, beep_graph **out)
+{
+    return guarded([&] {
+        if ((!src || !dst) && n) throw Error(BEEP_EINVAL, "null edge arrays");
+        if (!out) throw Error(BEEP_EINVAL, "null out");
+        *out = new beep_graph{beep::csr_from_edges(src, dst, n)};
+    });
+}
+
+int beep_graph_rmat(uint32_t scale, uint64_t samples, double a, double b, double c, uint64_t seed,
+                    uint32_t n_keep, beep_graph **out)
+{
+    return guarded([&] {
+        if (!out) throw Error(BEEP_EINVAL, "null out");
+        *out = new beep_graph{beep::gen_rmat(scale, samples, a, b, c, seed, n_keep)};
+    });
+}
+
+int beep_graph_chung_lu(uint32_t n, uint64_t samples, double gamma, double i0, uint64_t seed,
+                        beep_graph **out)
+{
+    return guarded([&] {
+        if (!out) throw Error(BEEP_EINVAL, "null out");
+        *out = new beep_graph{beep::gen_chung_lu(n, samples, gamma, i0, seed)};
+    });
+}
+
+int beep_graph_describe(const beep_graph *g, beep_graph_desc *o)
+{
+    return guarded([&] {
+        if (!g || !o) throw Error(BEEP_EINVAL, "null argument");
+        const auto &G = *g->g;
+        o->n = G.n;
+        o->m = G.m;
+        o->max_degree = G.max_degree;
+        o->cutoff = G.cutoff;
+        o->mean = G.mean;
+        o->sd = G.sd;
+        o->rowptr = G.rowptr.data();
+        o->col = G.col.data();
+    });
+}
+
+int beep_graph_write_bel(const beep_graph *g, const char *path)
+{
+    return guarded([&] {
+        if (!g || !path) throw Error(BEEP_EINVAL, "null argument");
+        beep::write_bel(*g->g, path);
+    });
+}
+
+void beep_graph_free(beep_graph *g) { delete g; }
+
+int beep_plan_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t n, int mode, beep_plan **out)
+{
+    return guarded([&] {
+        if (!src || !dst || !out) throw Error(BEEP_EINVAL, "null argument");
+        *out = new beep_plan{beep::plan_from_edges(src, dst, n, mode)};
+    });
+}
+
+int beep_plan_from_mtx(const char *path, int mode, beep_plan **out)
+{
+    return guarded([&] {
+        if (!path || !out) throw Error(BEEP_EINVAL, "null argument");
+        std::vector<uint32_t> s, d;
+        beep::read_edges(path, s, d);
+        *out = new beep_plan{beep::plan_from_edges(s.data(), d.data(), s.size(), mode)};
+    });
+}
+
+int beep_plan_describe(const beep_plan *p, beep_plan_desc *o)
+{
+    return guarded([&] {
+        if (!p || !o) throw Error(BEEP_EINVAL, "null argument");
+        const auto &P = *p->p;
+        o->k = P.k;
+        o->mode = P.mode;
+        o->lunmat = P.lunmat;
+        o->asc = P.asc;
+        o->max_qdeg = P.max_qdeg;
+        o->min_qdeg = P.min_qdeg;
+        memcpy(o->seq, P.seq, sizeof(P.seq));
+        memcpy(o->qdeg, P.qdeg, sizeof(P.qdeg));
+        memcpy(o->qedge_ptr, P.qedge_ptr, sizeof(P.qedge_ptr));
+        memcpy(o->qedge, P.qedge, sizeof(P.qedge));
+        memcpy(o->sym_ptr, P.sym_ptr, sizeof(P.sym_ptr));
+        memcpy(o->sym, P.sym, sizeof(P.sym));
+    });
+}
+
+void beep_plan_free(beep_plan *p) { delete p; }
+
+int beep_prepare(const beep_graph *g, const beep_plan *p, beep_prep **out)
+{
+    return guarded([&] {
+        if (!g || !p || !out) throw Error(BEEP_EINVAL, "null argument");
+        *out = new beep_prep{beep::prepare(*g->g, *p->p)};
+    });
+}
+
+int beep_prep_describe(const beep_prep *pp, beep_prep_desc *o)
+{
+    return guarded([&] {
+        if (!pp || !o) throw Error(BEEP_EINVAL, "null argument");
+        const auto &P = *pp->pp;
+        o->n = P.n;
+        o->m = P.m;
+        o->n_roots = P.roots.size();
+        o->q0_size = P.q0_size;
+        o->boundary = P.boundary;
+        o->cutoff = P.cutoff;
+        o->max_degree = P.max_degree;
+        o->max_root_degree = P.max_root_degree;
+        o->peel_rounds = P.peel_rounds;
+        o->asc = P.asc;
+        o->alg_bytes = P.alg_bytes;
+        o->n_hub_roots = P.n_hub;
+        o->n_lds_batches = P.batches.size();
+        o->rowptr = P.rowptr.data();
+        o->col = P.col.data();
+        o->ocol = P.ocol.data();
+        o->nlow = P.nlow.data();
+        o->roots = P.roots.data();
+    });
+}
+
+int beep_prep_upload(beep_prep *pp, int device)
+{
+    return guarded([&] {
+        if (!pp) throw Error(BEEP_EINVAL, "null argument");
+        beep::upload_prep(*pp->pp, device);
+    });
+}
+
+void beep_prep_free(beep_prep *pp) { delete pp; }
+
+void beep_opts_default(beep_opts *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->emit = BEEP_EMIT_NONE;
+    o->lanes = 0;
+    o->donate = 0;
+    o->donate_min = 64;
+    o->timing = 1;
+}
+
+int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_result *out)
+{
+    return guarded([&] {
+        if (!pp || !p || !out) throw Error(BEEP_EINVAL, "null argument");
+        beep_opts def;
+        beep_opts_default(&def);
+        const beep_opts &opt = o ? *o : def;
+        if (opt.emit < BEEP_EMIT_NONE || opt.emit > BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "bad emit mode");
+        if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
+            throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
+        memset(out, 0, sizeof(*out));
+        beep::count_on_device(*pp->pp, *p->p, opt, *out);
+    });
+}
+
+uint64_t beep_tuple_hash(const uint32_t *t, int len)
+{
+    auto mix = [](uint64_t z) {
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    uint64_t h = 0x6a09e667f3bcc909ull ^ (uint64_t)len;
+    for (int i = 0; i < len; i++) h = mix(h + 0x9e3779b97f4a7c15ull * (uint64_t)(i + 1) + t[i]);
+    return h;
+}
+
+}  // extern "C"

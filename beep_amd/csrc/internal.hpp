@@ -1,0 +1,147 @@
+// internal.hpp -- host-side data structures shared by the libbeep sources.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/beep.h"
+
+namespace beep {
+
+constexpr int kMaxK = BEEP_MAXK;
+
+// Error carrying an errno-style code; the C ABI converts it into a return value
+// and the thread's last-error message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+// ---------------------------------------------------------------- threading
+unsigned host_threads();
+
+// Dynamic parallel loop over [0, n) in chunks of `grain`.
+template <class F>
+void parallel_for(size_t n, F &&f, size_t grain = 1 << 14)
+{
+    unsigned T = host_threads();
+    if (n <= grain || T <= 1) {
+        if (n) f(size_t(0), n);
+        return;
+    }
+    size_t chunks = (n + grain - 1) / grain;
+    if (T > chunks) T = (unsigned)chunks;
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            size_t b = next.fetch_add(grain);
+            if (b >= n) break;
+            f(b, b + grain < n ? b + grain : n);
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (unsigned t = 1; t < T; ++t) th.emplace_back(worker);
+    worker();
+    for (auto &x : th) x.join();
+}
+
+// ---------------------------------------------------------------- graph
+struct Graph {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    std::vector<uint64_t> rowptr;  // n + 1
+    std::vector<uint32_t> col;     // m
+    uint32_t max_degree = 0;
+    uint32_t cutoff = 0;
+    double mean = 0, sd = 0;
+    bool rows_sorted = true;       // every row ascending by id
+    void finalize();               // degree stats + cutoff
+};
+
+// Edge list -> CSR with CSRCOO::from_edgelist rules (CSRCOO.cuh:37-80).
+std::unique_ptr<Graph> csr_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t ne);
+// Reads .bel/.tsv/.mtx edges with the reference's byte semantics.
+void read_edges(const std::string &path, std::vector<uint32_t> &src, std::vector<uint32_t> &dst);
+// cutoff = ceil(mean + 6 sd) (utils.cuh:186-261).
+uint32_t compute_cutoff(const Graph &g, double *mean, double *sd);
+void write_bel(const Graph &g, const std::string &path);
+
+std::unique_ptr<Graph> gen_rmat(uint32_t scale, uint64_t samples, double a, double b, double c,
+                                uint64_t seed, uint32_t n_keep);
+std::unique_ptr<Graph> gen_chung_lu(uint32_t n, uint64_t samples, double gamma, double i0,
+                                    uint64_t seed);
+// Sorted unique undirected edge keys (u<v packed as u<<32|v) -> symmetric CSR.
+std::unique_ptr<Graph> csr_from_undirected_keys(std::vector<uint64_t> &keys, uint32_t n);
+
+// ---------------------------------------------------------------- plan
+struct Plan {
+    int k = 0, mode = BEEP_MODE_SGM, lunmat = 0, asc = 0, max_qdeg = 0, min_qdeg = 0;
+    uint32_t seq[kMaxK] = {}, qdeg[kMaxK] = {};
+    uint32_t qedge_ptr[kMaxK + 1] = {}, qedge[kMaxK * (kMaxK - 1) / 2] = {};
+    uint32_t sym_ptr[kMaxK + 1] = {}, sym[kMaxK * (kMaxK - 1) / 2] = {};
+    // derived, used by the kernels
+    int L = 0;            // last materialised level + 1 = k - lunmat
+    int skip_split = 0;   // SYMNODE_PTR[2] == 1 (sgm_kernelsLD.cuh:60)
+    uint16_t qmask[kMaxK] = {};  // positions QEDGE(l)[1:]
+    uint16_t smask[kMaxK] = {};  // positions s in SYM(l), s > 0
+};
+
+std::unique_ptr<Plan> plan_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t ne,
+                                      int mode);
+
+// ---------------------------------------------------------------- prep
+struct DevPrep;  // device mirror (sgm_engine.hip)
+
+// One LDS-resident batch of consecutive roots.
+struct LdsBatch {
+    uint32_t first;   // index into roots
+    uint32_t count;   // number of roots
+};
+
+struct Prep {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    int asc = 0, min_qdeg = 0, max_qdeg = 0, peel_rounds = 0;
+    uint32_t cutoff = 0, max_degree = 0, max_root_degree = 0;
+    std::vector<uint64_t> rowptr;  // peeled
+    std::vector<uint32_t> col;     // peeled, id ascending
+    std::vector<uint32_t> ocol;    // oriented
+    std::vector<uint32_t> nlow;    // #smaller keys
+    std::vector<uint32_t> hlen;    // length of the bitmap-build scan range of each row
+    std::vector<uint32_t> roots;   // R
+    uint64_t q0_size = 0, boundary = 0;
+    uint64_t alg_bytes = 0;
+
+    // Tiering: roots[0, n_hub) have their bitmap in HBM, the rest are packed
+    // into LDS batches.  Within each tier roots keep the descending order.
+    uint32_t n_hub = 0;
+    std::vector<LdsBatch> batches;
+    std::vector<uint8_t> batch_lanes_log2;  // per batch: log2 of DFS-group lanes
+
+    DevPrep *dev = nullptr;
+    ~Prep();
+};
+
+std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p);
+
+// LDS budget of the batch kernel (words / slots); shared with the kernels.
+struct LdsBudget {
+    static constexpr uint32_t kMaxRoots = 128;     // roots per batch
+    static constexpr uint32_t kMaxSumDeg = 1024;   // sum of degrees per batch
+    static constexpr uint32_t kMaxEWords = 4096;   // bitmap words (u64)
+    static constexpr uint32_t kMaxHash = 2048;     // hash slots
+};
+
+uint32_t hash_slots_for(uint32_t d);
+void build_tiers(Prep &pp);
+
+}  // namespace beep

@@ -1,0 +1,238 @@
+// prep.cpp -- data preparation on the host: peeling, degree orientation, the
+// root queue, and the split of roots into LDS batches and HBM hub roots.
+//
+// Semantics follow SG_Match::peel_data (subgraph_matching.cuh:685-764),
+// degree_ordering (:1282-1336 with host_utils.cuh:89-134, 230-246) and the
+// root selection of count_subgraphs (:1080-1131 with GraphQueue.cuh:85-120,
+// device_utils.cuh:60-97).  The reference runs these as a chain of
+// synchronous kernels over managed memory; here they are parallel host loops
+// whose output is uploaded once and stays resident in HBM.
+
+#include <algorithm>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace beep {
+
+uint32_t hash_slots_for(uint32_t d)
+{
+    uint32_t h = 2;
+    while (h < 2 * d) h <<= 1;
+    return h;
+}
+
+namespace {
+
+// Peel: S = {v : 1 <= d(v) < min_qdeg}; every edge at S is removed in both
+// directions (the reverse entry is the first occurrence of v in the
+// neighbour's row, common_utils.cuh:217-246); repeat while |S| > 0.05 n with
+// n fixed (subgraph_matching.cuh:758).
+int peel(const Graph &g, int min_qdeg, std::vector<uint64_t> &rp, std::vector<uint32_t> &col)
+{
+    const uint32_t n = g.n;
+    rp = g.rowptr;
+    col = g.col;
+    std::vector<uint32_t> deg(n);
+    parallel_for(n, [&](size_t b, size_t e) {
+        for (size_t v = b; v < e; v++) deg[v] = (uint32_t)(rp[v + 1] - rp[v]);
+    });
+    const uint32_t lim = (uint32_t)std::max(min_qdeg, 0);
+    int rounds = 0;
+    std::vector<uint8_t> keep;
+    for (;;) {
+        rounds++;
+        std::vector<uint32_t> S;
+        for (uint32_t v = 0; v < n; v++)
+            if (deg[v] >= 1 && deg[v] < lim) S.push_back(v);
+        const uint64_t m = rp[n];
+        keep.assign(m, 1);
+        const bool sorted = g.rows_sorted;
+        parallel_for(S.size(), [&](size_t b, size_t e) {
+            for (size_t s = b; s < e; s++) {
+                uint32_t v = S[s];
+                for (uint64_t x = rp[v]; x < rp[v + 1]; x++) {
+                    __atomic_store_n(&keep[x], (uint8_t)0, __ATOMIC_RELAXED);
+                    uint32_t w = col[x];
+                    const uint32_t *row = col.data() + rp[w];
+                    const uint64_t len = rp[w + 1] - rp[w];
+                    uint64_t f = len;
+                    if (sorted) {
+                        f = (uint64_t)(std::lower_bound(row, row + len, v) - row);
+                        if (f < len && row[f] != v) f = len;
+                    } else {
+                        for (uint64_t y = 0; y < len; y++)
+                            if (row[y] == v) { f = y; break; }
+                    }
+                    if (f < len) __atomic_store_n(&keep[rp[w] + f], (uint8_t)0, __ATOMIC_RELAXED);
+                }
+            }
+        }, 256);
+        std::vector<uint64_t> nrp(n + 1, 0);
+        parallel_for(n, [&](size_t b, size_t e) {
+            for (size_t v = b; v < e; v++) {
+                uint64_t c = 0;
+                for (uint64_t x = rp[v]; x < rp[v + 1]; x++) c += keep[x];
+                nrp[v + 1] = c;
+            }
+        });
+        for (uint32_t v = 0; v < n; v++) nrp[v + 1] += nrp[v];
+        std::vector<uint32_t> ncol(nrp[n]);
+        parallel_for(n, [&](size_t b, size_t e) {
+            for (size_t v = b; v < e; v++) {
+                uint64_t o = nrp[v];
+                for (uint64_t x = rp[v]; x < rp[v + 1]; x++)
+                    if (keep[x]) ncol[o++] = col[x];
+                deg[v] = (uint32_t)(nrp[v + 1] - nrp[v]);
+            }
+        });
+        rp.swap(nrp);
+        col.swap(ncol);
+        if (!((double)S.size() > 0.05 * (double)n)) break;
+    }
+    return rounds;
+}
+
+}  // namespace
+
+std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
+{
+    auto pp = std::make_unique<Prep>();
+    pp->n = g.n;
+    pp->asc = p.asc;
+    pp->min_qdeg = p.min_qdeg;
+    pp->max_qdeg = p.max_qdeg;
+    pp->cutoff = g.cutoff;
+    pp->peel_rounds = peel(g, p.min_qdeg, pp->rowptr, pp->col);
+    const uint32_t n = g.n;
+    pp->m = pp->rowptr[n];
+    const auto &rp = pp->rowptr;
+    std::vector<uint32_t> deg(n);
+    parallel_for(n, [&](size_t b, size_t e) {
+        for (size_t v = b; v < e; v++) deg[v] = (uint32_t)(rp[v + 1] - rp[v]);
+    });
+    // Orientation: key(v) = (deg(v), v).  asc rows ascend by key; desc rows
+    // descend by degree with ids ascending (the stable sorts of :1305-1331 on an
+    // id-ascending input).  nlow(u) = #neighbours with a smaller key.
+    pp->ocol.resize(pp->m);
+    pp->nlow.assign(n, 0);
+    pp->hlen.assign(n, 0);
+    const bool asc = p.asc != 0;
+    parallel_for(n, [&](size_t b, size_t e) {
+        for (size_t u = b; u < e; u++) {
+            const uint64_t rb = rp[u], re = rp[u + 1];
+            uint32_t du = deg[u], lo = 0;
+            for (uint64_t x = rb; x < re; x++) {
+                uint32_t w = pp->col[x];
+                if (deg[w] < du || (deg[w] == du && w < (uint32_t)u)) lo++;
+                pp->ocol[x] = w;
+            }
+            pp->nlow[u] = lo;
+            // Bitmap-build scan range: every adjacent pair {v, w} is found from
+            // exactly one endpoint.  asc rows: the suffix of larger keys.  desc
+            // rows (degree descending, ids ascending): the prefix of larger
+            // degrees followed by equal degrees with smaller ids.
+            if (asc) {
+                pp->hlen[u] = du - lo;
+            } else {
+                uint32_t c = 0;
+                for (uint64_t x = rb; x < re; x++) {
+                    uint32_t w = pp->col[x];
+                    if (deg[w] > du || (deg[w] == du && w < (uint32_t)u)) c++;
+                }
+                pp->hlen[u] = c;
+            }
+            uint32_t *row = pp->ocol.data() + rb;
+            if (asc)
+                std::sort(row, row + (re - rb), [&](uint32_t a, uint32_t c) {
+                    return deg[a] != deg[c] ? deg[a] < deg[c] : a < c;
+                });
+            else
+                std::sort(row, row + (re - rb), [&](uint32_t a, uint32_t c) {
+                    return deg[a] != deg[c] ? deg[a] > deg[c] : a < c;
+                });
+        }
+    }, 4096);
+    uint32_t maxd = 0;
+    for (uint32_t v = 0; v < n; v++) maxd = std::max(maxd, deg[v]);
+    pp->max_degree = maxd;
+    // Roots: Q0 = {max_qdeg <= d < 100 cutoff} ordered by degree descending
+    // with ids ascending; boundary = #{d > cutoff}; the head of Q0 is skipped
+    // when ascending and boundary >= 1 (work_list_head1 starts at 1).
+    const uint32_t hi = g.cutoff * 100u;
+    const uint32_t lo = (uint32_t)p.max_qdeg;
+    std::vector<uint64_t> bucket((size_t)maxd + 2, 0);
+    uint64_t nq = 0;
+    for (uint32_t v = 0; v < n; v++)
+        if (deg[v] >= lo && deg[v] < hi) { bucket[maxd - deg[v]]++; nq++; }
+    uint64_t s = 0;
+    for (auto &c : bucket) { uint64_t t = c; c = s; s += t; }
+    std::vector<uint32_t> q0(nq);
+    for (uint32_t v = 0; v < n; v++)
+        if (deg[v] >= lo && deg[v] < hi) q0[bucket[maxd - deg[v]]++] = v;
+    uint64_t bnd = 0;
+    while (bnd < nq && deg[q0[bnd]] > g.cutoff) bnd++;
+    const uint64_t skip = (asc && bnd >= 1) ? 1 : 0;
+    pp->roots.assign(q0.begin() + (ptrdiff_t)skip, q0.end());
+    pp->q0_size = nq;
+    pp->boundary = bnd;
+    pp->max_root_degree = pp->roots.empty() ? 0 : deg[pp->roots[0]];
+    // B_alg = sum over roots of 4(d(u)+2) + sum over neighbours 4(d(v)+2).
+    std::atomic<uint64_t> bytes{0};
+    parallel_for(pp->roots.size(), [&](size_t b, size_t e) {
+        uint64_t acc = 0;
+        for (size_t r = b; r < e; r++) {
+            uint32_t u = pp->roots[r];
+            acc += 4ull * (deg[u] + 2);
+            for (uint64_t x = rp[u]; x < rp[u + 1]; x++) acc += 4ull * (deg[pp->col[x]] + 2);
+        }
+        bytes.fetch_add(acc);
+    }, 1024);
+    pp->alg_bytes = bytes.load();
+    build_tiers(*pp);
+    return pp;
+}
+
+// Roots whose d x ceil(d/64) bitmap fits the batch kernel's LDS budget are
+// packed greedily (in queue order) into batches; larger roots keep their
+// bitmap in HBM and are processed by the hub kernels.
+void build_tiers(Prep &pp)
+{
+    const auto &rp = pp.rowptr;
+    auto deg = [&](uint32_t u) { return (uint32_t)(rp[u + 1] - rp[u]); };
+    auto fits_lds = [&](uint32_t d) {
+        uint64_t w = (d + 63) / 64;
+        return (uint64_t)d * w <= LdsBudget::kMaxEWords && d <= LdsBudget::kMaxSumDeg &&
+               hash_slots_for(d) <= LdsBudget::kMaxHash;
+    };
+    uint32_t nh = 0;
+    while (nh < pp.roots.size() && !fits_lds(deg(pp.roots[nh]))) nh++;
+    pp.n_hub = nh;
+    pp.batches.clear();
+    pp.batch_lanes_log2.clear();
+    uint32_t i = nh;
+    const uint32_t nr = (uint32_t)pp.roots.size();
+    while (i < nr) {
+        LdsBatch b{i, 0};
+        uint64_t sd = 0, sw = 0, sh = 0;
+        const uint32_t w0 = (deg(pp.roots[i]) + 63) / 64;
+        while (i < nr && b.count < LdsBudget::kMaxRoots) {
+            uint32_t d = deg(pp.roots[i]);
+            uint64_t w = (d + 63) / 64;
+            if (b.count && (sd + d > LdsBudget::kMaxSumDeg || sw + d * w > LdsBudget::kMaxEWords ||
+                            sh + hash_slots_for(d) > LdsBudget::kMaxHash))
+                break;
+            sd += d;
+            sw += d * w;
+            sh += hash_slots_for(d);
+            b.count++;
+            i++;
+        }
+        uint8_t lg = 0;
+        while ((1u << lg) < w0) lg++;
+        pp.batches.push_back(b);
+        pp.batch_lanes_log2.push_back(lg);
+    }
+}
+
+}  // namespace beep

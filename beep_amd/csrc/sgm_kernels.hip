@@ -1,0 +1,700 @@
+// sgm_kernels.hip -- CDNA4 (gfx950) kernels of the enumeration path.
+//
+// The reference runs one persistent CUDA kernel in which a 320-thread block
+// claims a root, builds the root's d x ceil(d/32) neighbourhood bitmap in
+// global memory by binary search, and 16-lane partitions walk the DFS over
+// 32-bit words (sgm_kernelsLD.cuh:7-273, device_utils.cuh:60-456).
+//
+// Here the same enumeration is organised for wave64:
+//  * bitmap words are 64 bits; a DFS group is P lanes of one wave (P = 1..64,
+//    lane i of a group owns words i, i+P, ...), the next candidate is found
+//    with one wave ballot over the group's words and one ds_bpermute;
+//  * roots whose bitmap fits are packed into LDS batches: a workgroup stages
+//    the rows, an LDS hash (vertex -> row index) and the bitmaps, then its
+//    groups claim (root, x_1) items from an LDS counter;
+//  * the bitmap is built by streaming, for every row vertex v, only the part
+//    of v's oriented row that holds neighbours with a larger (degree, id) key
+//    and probing the hash -- each adjacent pair inside N(u) is found once and
+//    both bits are set -- instead of d^2/2 binary searches;
+//  * roots too large for LDS ("hubs") keep bitmap + hash in an HBM arena built
+//    by grid-wide kernels, and every wave of the grid takes (hub, x_1) items,
+//    so one hub is shared by the whole GPU.
+//
+// Candidate sets follow count_tri (device_utils.cuh:179-209) for level 2 and
+// compute_intersection (:426-456) for deeper levels; counting stops at level
+// L-1 = k-1-LUNMAT as backtrack does (:294-326).
+
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "sgm_kernels.hpp"
+
+namespace beep {
+
+// ------------------------------------------------------------------ helpers
+
+// bits [0, d) of word w
+__device__ __forceinline__ uint64_t valid_mask(uint32_t w, uint32_t d)
+{
+    const uint32_t lo = w << 6;
+    if (lo + 64 <= d) return ~0ull;
+    if (lo >= d) return 0ull;
+    return (1ull << (d - lo)) - 1ull;
+}
+
+// bits > x of word w (explicit branches: shifts by 64 are never issued)
+__device__ __forceinline__ uint64_t gt_mask(uint32_t x, uint32_t w)
+{
+    const uint32_t xw = x >> 6;
+    if (xw > w) return 0ull;
+    if (xw < w) return ~0ull;
+    const uint32_t b = x & 63u;
+    return b == 63u ? 0ull : (~0ull << (b + 1u));
+}
+
+__device__ __forceinline__ uint64_t bit_at(uint32_t x, uint32_t w)
+{
+    return (x >> 6) == w ? (1ull << (x & 63u)) : 0ull;
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t v, uint32_t H)
+{
+    uint32_t h = v * 0x9E3779B1u;
+    h ^= h >> 15;
+    return h & (H - 1u);
+}
+
+__host__ __device__ __forceinline__ uint32_t hash_slots_dev(uint32_t d)
+{
+    uint32_t h = 2;
+    while (h < 2 * d) h <<= 1;
+    return h;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t hash_step(uint64_t h, int pos, uint32_t v)
+{
+    return mix64(h + 0x9e3779b97f4a7c15ull * (uint64_t)(pos + 1) + v);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_xor_u64(uint64_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Wave inclusive scan.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// Flushes per-lane accumulators: one atomic per wave and counter.
+__device__ __forceinline__ void flush_acc(const KOut &out, uint64_t cnt, uint64_t hs, uint64_t hx)
+{
+    cnt = wave_sum_u64(cnt);
+    hs = wave_sum_u64(hs);
+    hx = wave_xor_u64(hx);
+    if ((threadIdx.x & 63) == 0) {
+        if (cnt) atomicAdd(&out.acc[0], (unsigned long long)cnt);
+        if (hs) atomicAdd(&out.acc[1], (unsigned long long)hs);
+        if (hx) atomicXor(&out.acc[2], (unsigned long long)hx);
+    }
+}
+
+// Emits the tuples (u, A[x_1], ..., A[x_{l-1}], A[i]) for every bit i of word
+// w of the last level: hash accumulation and optional buffer append.
+template <class XGet>
+__device__ __forceinline__ void emit_word(const KOut &out, const KPlan &pl, int l, uint32_t u,
+                                          const uint32_t *A, uint64_t t, uint32_t w, XGet xget,
+                                          uint64_t &hs, uint64_t &hx)
+{
+    const int len = l + 1;
+    uint64_t h = 0x6a09e667f3bcc909ull ^ (uint64_t)len;
+    h = hash_step(h, 0, u);
+    for (int q = 1; q < l; q++) h = hash_step(h, q, A[xget(q)]);
+    uint64_t base = 0;
+    if (out.buf) base = atomicAdd(&out.acc[3], (unsigned long long)__popcll(t));
+    uint64_t idx = 0;
+    while (t) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(t);
+        t &= t - 1;
+        const uint32_t vi = A[(w << 6) + b];
+        const uint64_t hv = hash_step(h, l, vi);
+        hs += hv;
+        hx ^= hv;
+        if (out.buf && base + idx < out.cap) {
+            uint32_t *dst = out.buf + (base + idx) * (uint64_t)len;
+            dst[0] = u;
+            for (int q = 1; q < l; q++) dst[q] = A[xget(q)];
+            dst[l] = vi;
+        }
+        idx++;
+    }
+}
+
+// ------------------------------------------------------------ LDS batches
+
+struct LdsLayout {
+    uint32_t oE, oHBeg, oStk, oA, oPre, oHKey, oRoot, oHVal, oX, oJRoot, oMisc, bytes;
+    uint32_t ewords, nlev;
+};
+
+__host__ __device__ inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+__host__ inline LdsLayout make_lds_layout(uint32_t ewords, uint32_t nlev)
+{
+    LdsLayout L{};
+    uint32_t o = 0;
+    L.ewords = ewords;
+    L.nlev = nlev;
+    L.oE = o;       o += ewords * 8;
+    L.oHBeg = o;    o += LdsBudget::kMaxSumDeg * 8;
+    L.oStk = o;     o += nlev * kBlock * 8;
+    L.oRoot = o;    o += LdsBudget::kMaxRoots * (uint32_t)sizeof(RootRec);
+    L.oA = o;       o += LdsBudget::kMaxSumDeg * 4;
+    L.oPre = o;     o += align_up((LdsBudget::kMaxSumDeg + 1) * 4, 8);
+    L.oHKey = o;    o += LdsBudget::kMaxHash * 4;
+    L.oMisc = o;    o += 32 * 4;
+    L.oHVal = o;    o += LdsBudget::kMaxHash * 2;
+    L.oX = o;       o += kMaxLevels * kBlock * 2;
+    L.oJRoot = o;   o += LdsBudget::kMaxSumDeg;
+    L.bytes = align_up(o, 16);
+    return L;
+}
+
+// Misc slots
+enum { M_BATCH = 0, M_NEXT = 1, M_QM = 4, M_SM = 14 };
+
+// Block-wide exclusive scan of a[0..n) in place (n <= 4 * kBlock); a[n] = total.
+__device__ void block_excl_scan(uint32_t *a, uint32_t n, uint32_t *tmp4)
+{
+    const int tid = threadIdx.x;
+    const int per = (int)((n + kBlock - 1) / kBlock);
+    const uint32_t b = (uint32_t)(tid * per);
+    uint32_t loc = 0;
+    for (int i = 0; i < per; i++)
+        if (b + i < n) loc += a[b + i];
+    uint32_t inc = wave_incl_scan(loc);
+    const int wv = tid >> 6;
+    if ((tid & 63) == 63) tmp4[wv] = inc;
+    __syncthreads();
+    uint32_t wofs = 0;
+    for (int i = 0; i < wv; i++) wofs += tmp4[i];
+    uint32_t run = wofs + inc - loc;
+    for (int i = 0; i < per; i++)
+        if (b + i < n) {
+            uint32_t x = a[b + i];
+            a[b + i] = run;
+            run += x;
+        }
+    if (tid == kBlock - 1) a[n] = run;
+    __syncthreads();
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
+                                                           const RootRec *__restrict__ recs,
+                                                           const uint32_t *__restrict__ batch_first,
+                                                           const uint32_t *__restrict__ batch_count,
+                                                           uint32_t b0, uint32_t b1,
+                                                           unsigned int *cursor, LdsLayout lay,
+                                                           KOut out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint64_t *sE = (uint64_t *)(smem + lay.oE);
+    uint64_t *sHBeg = (uint64_t *)(smem + lay.oHBeg);
+    uint64_t *sStk = (uint64_t *)(smem + lay.oStk);
+    RootRec *sRoot = (RootRec *)(smem + lay.oRoot);
+    uint32_t *sA = (uint32_t *)(smem + lay.oA);
+    uint32_t *sPre = (uint32_t *)(smem + lay.oPre);
+    uint32_t *sHKey = (uint32_t *)(smem + lay.oHKey);
+    uint32_t *sMisc = (uint32_t *)(smem + lay.oMisc);
+    uint16_t *sHVal = (uint16_t *)(smem + lay.oHVal);
+    uint16_t *sX = (uint16_t *)(smem + lay.oX);
+    uint8_t *sJRoot = (uint8_t *)(smem + lay.oJRoot);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int lx = lane & (P - 1);
+    const int pbase = lane & ~(P - 1);
+    const int L = pl.L;
+
+    if (tid < kMaxLevels) {
+        sMisc[M_QM + tid] = pl.qmask[tid];
+        sMisc[M_SM + tid] = pl.smask[tid];
+    }
+
+    uint64_t cnt = 0, hs = 0, hx = 0;
+    auto xget = [&](int q) -> uint32_t { return sX[q * kBlock + tid]; };
+
+    for (;;) {
+        if (tid == 0) sMisc[M_BATCH] = b0 + atomicAdd(cursor, 1u);
+        __syncthreads();
+        const uint32_t b = sMisc[M_BATCH];
+        if (b >= b1) break;
+        const uint32_t first = batch_first[b], nr = batch_count[b];
+        if (tid < (int)nr) sRoot[tid] = recs[first + tid];
+        __syncthreads();
+        const RootRec last = sRoot[nr - 1];
+        const uint32_t sumd = last.doff + last.d;
+        const uint32_t ew = last.eoff + last.d * ((last.d + 63) >> 6);
+        const uint32_t hsl = last.hoff + hash_slots_dev(last.d);
+        for (uint32_t i = tid; i < ew; i += kBlock) sE[i] = 0;
+        for (uint32_t i = tid; i < hsl; i += kBlock) sHKey[i] = 0;
+        if (tid == 0) sMisc[M_NEXT] = 0;
+        __syncthreads();
+
+        // rows, hash tables and "higher" ranges of every row vertex
+        for (uint32_t j = tid; j < sumd; j += kBlock) {
+            uint32_t lo = 0, hi = nr;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (sRoot[mid].doff <= j) lo = mid; else hi = mid;
+            }
+            const RootRec rr = sRoot[lo];
+            const uint32_t jl = j - rr.doff;
+            const uint32_t v = g.ocol[rr.rbeg + jl];
+            sA[j] = v;
+            sJRoot[j] = (uint8_t)lo;
+            const uint32_t H = hash_slots_dev(rr.d);
+            uint32_t s = hash_slot(v, H);
+            for (;;) {
+                if (atomicCAS(&sHKey[rr.hoff + s], 0u, v + 1u) == 0u) {
+                    sHVal[rr.hoff + s] = (uint16_t)jl;
+                    break;
+                }
+                s = (s + 1) & (H - 1);
+            }
+            sHBeg[j] = g.rowptr[v] + (g.asc ? g.nlow[v] : 0u);
+            sPre[j] = g.hlen[v];
+        }
+        __syncthreads();
+        block_excl_scan(sPre, sumd, sMisc + 28);
+        const uint32_t total = sPre[sumd];
+
+        // bitmap: each adjacent pair (v, w), key(v) < key(w), is found from v
+        for (uint32_t e = tid; e < total; e += kBlock) {
+            uint32_t lo = 0, hi = sumd;  // largest j with sPre[j] <= e
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (sPre[mid] <= e) lo = mid; else hi = mid;
+            }
+            const uint32_t j = lo;
+            const RootRec rr = sRoot[sJRoot[j]];
+            const uint32_t jl = j - rr.doff;
+            const uint32_t w = g.ocol[sHBeg[j] + (e - sPre[j])];
+            const uint32_t H = hash_slots_dev(rr.d);
+            const uint32_t W = (rr.d + 63) >> 6;
+            uint32_t s = hash_slot(w, H);
+            for (;;) {
+                const uint32_t key = sHKey[rr.hoff + s];
+                if (key == 0u) break;
+                if (key == w + 1u) {
+                    const uint32_t i = sHVal[rr.hoff + s];
+                    atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
+                    atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                }
+                s = (s + 1) & (H - 1);
+            }
+        }
+        __syncthreads();
+
+        // DFS over (root, x_1) items
+        for (;;) {
+            uint32_t item = 0;
+            if (lx == 0) item = atomicAdd(&sMisc[M_NEXT], 1u);
+            item = __shfl(item, 0, P);
+            if (item >= sumd) break;
+            const RootRec rr = sRoot[sJRoot[item]];
+            const uint32_t jl = item - rr.doff;
+            if (pl.skip_split && jl < rr.split) continue;
+            if (L < 3) continue;
+            const uint32_t d = rr.d, W = (d + 63) >> 6;
+            const uint64_t *E = sE + rr.eoff;
+            const uint32_t *A = sA + rr.doff;
+            const uint32_t w = (uint32_t)lx;
+            const bool own = w < W;
+            sX[1 * kBlock + tid] = (uint16_t)jl;
+
+            auto level_word = [&](int l) -> uint64_t {
+                uint64_t t = valid_mask(w, d);
+                uint32_t qm = sMisc[M_QM + l];
+                while (qm) {
+                    const int q = __builtin_ctz(qm);
+                    qm &= qm - 1;
+                    t &= E[xget(q) * W + w];
+                }
+                for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
+                uint32_t sm = sMisc[M_SM + l];
+                while (sm) {
+                    const int s = __builtin_ctz(sm);
+                    sm &= sm - 1;
+                    t &= gt_mask(xget(s), w);
+                }
+                return t;
+            };
+
+            const uint64_t t2 = own ? level_word(2) : 0ull;
+            if (L == 3) {
+                cnt += (uint64_t)__popcll(t2);
+                if (pl.emit && t2) emit_word(out, pl, 2, rr.u, A, t2, w, xget, hs, hx);
+                continue;
+            }
+            sStk[tid] = t2;
+            int l = 3;
+            uint32_t pos = 0;
+            for (;;) {
+                uint64_t v = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
+                const uint32_t pw = pos >> 6;
+                if (w < pw) v = 0ull;
+                else if (w == pw) v &= (~0ull << (pos & 63u));
+                const uint64_t bal = __ballot(v != 0ull);
+                const uint64_t mine = (P == 64) ? bal : ((bal >> pbase) & ((1ull << P) - 1ull));
+                if (mine == 0ull) {
+                    if (l == 3) break;
+                    --l;
+                    pos = (uint32_t)xget(l - 1) + 1u;
+                    continue;
+                }
+                const int f = __builtin_ctzll(mine);
+                const uint64_t vf = __shfl(v, f, P);
+                const uint32_t xi = ((uint32_t)f << 6) + (uint32_t)__builtin_ctzll(vf);
+                sX[(l - 1) * kBlock + tid] = (uint16_t)xi;
+                const uint64_t t = own ? level_word(l) : 0ull;
+                if (l == L - 1) {
+                    cnt += (uint64_t)__popcll(t);
+                    if (pl.emit && t) emit_word(out, pl, l, rr.u, A, t, w, xget, hs, hx);
+                    pos = xi + 1u;
+                } else {
+                    sStk[(l - 2) * kBlock + tid] = t;
+                    ++l;
+                    pos = 0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    flush_acc(out, cnt, hs, hx);
+}
+
+// ------------------------------------------------------------------- hubs
+
+// Hash of every hub row: vertex -> row index (linear probing, 0 = empty).
+__global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec *__restrict__ hubs,
+                                                          const uint32_t *__restrict__ hub_of_row,
+                                                          uint64_t g0, uint64_t g1,
+                                                          uint32_t *keys, uint32_t *vals)
+{
+    for (uint64_t r = g0 + blockIdx.x * (uint64_t)kBlock + threadIdx.x; r < g1;
+         r += (uint64_t)gridDim.x * kBlock) {
+        const HubRec h = hubs[hub_of_row[r]];
+        const uint32_t jl = (uint32_t)(r - h.joff);
+        const uint32_t v = g.ocol[h.rbeg + jl];
+        uint32_t s = hash_slot(v, h.hslots);
+        for (;;) {
+            if (atomicCAS(&keys[h.hoff + s], 0u, v + 1u) == 0u) {
+                vals[h.hoff + s] = jl;
+                break;
+            }
+            s = (s + 1) & (h.hslots - 1);
+        }
+    }
+}
+
+// Hub bitmaps: waves take chunks of 64 rows, scan the lengths of their
+// "higher" ranges and stream the concatenation lane-parallel.
+__global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubRec *__restrict__ hubs,
+                                                            const uint32_t *__restrict__ hub_of_row,
+                                                            uint64_t g0, uint64_t g1,
+                                                            const uint32_t *__restrict__ keys,
+                                                            const uint32_t *__restrict__ vals,
+                                                            uint64_t *E)
+{
+    __shared__ uint64_t sBeg[kBlock];
+    __shared__ uint32_t sPre[kBlock];
+    __shared__ uint32_t sHub[kBlock];
+    __shared__ uint32_t sJl[kBlock];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint64_t nchunks = (g1 - g0 + 63) / 64;
+    const uint64_t wstride = (uint64_t)gridDim.x * (kBlock / 64);
+    for (uint64_t c = blockIdx.x * (uint64_t)(kBlock / 64) + wv; c < nchunks; c += wstride) {
+        const uint64_t r = g0 + c * 64 + lane;
+        uint32_t len = 0;
+        if (r < g1) {
+            const uint32_t hi = hub_of_row[r];
+            const HubRec h = hubs[hi];
+            const uint32_t jl = (uint32_t)(r - h.joff);
+            const uint32_t v = g.ocol[h.rbeg + jl];
+            sBeg[tid] = g.rowptr[v] + (g.asc ? g.nlow[v] : 0u);
+            len = g.hlen[v];
+            sHub[tid] = hi;
+            sJl[tid] = jl;
+        }
+        const uint32_t inc = wave_incl_scan(len);
+        sPre[tid] = inc;
+        const uint32_t total = __shfl(inc, 63, 64);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int wb = wv * 64;
+        for (uint32_t e = lane; e < total; e += 64) {
+            int lo = 0, hi = 63;  // smallest k with pre[k] > e
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sPre[wb + mid] > e) hi = mid; else lo = mid + 1;
+            }
+            const uint32_t before = lo ? sPre[wb + lo - 1] : 0u;
+            const uint32_t wv2 = g.ocol[sBeg[wb + lo] + (e - before)];
+            const HubRec h = hubs[sHub[wb + lo]];
+            const uint32_t jl = sJl[wb + lo];
+            const uint64_t W = (h.d + 63) >> 6;
+            uint32_t s = hash_slot(wv2, h.hslots);
+            for (;;) {
+                const uint32_t key = keys[h.hoff + s];
+                if (key == 0u) break;
+                if (key == wv2 + 1u) {
+                    const uint32_t i = vals[h.hoff + s];
+                    atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)jl * W + (i >> 6)], 1ull << (i & 63));
+                    atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)i * W + (jl >> 6)], 1ull << (jl & 63));
+                }
+                s = (s + 1) & (h.hslots - 1);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Hub DFS: one wave per (hub, x_1) item; lane i owns words i, i+64, ...;
+// candidate stack in a per-wave HBM slot of nlev x wmax words.
+__global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
+                                                         const HubRec *__restrict__ hubs,
+                                                         const uint32_t *__restrict__ hub_of_row,
+                                                         uint64_t g0, uint64_t g1,
+                                                         unsigned long long *cursor,
+                                                         const uint64_t *__restrict__ E,
+                                                         uint64_t *stk, uint32_t wmax, KOut out)
+{
+    __shared__ uint32_t sX[kBlock / 64][kMaxLevels];
+    __shared__ uint32_t sQM[kMaxLevels], sSM[kMaxLevels];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid < kMaxLevels) {
+        sQM[tid] = pl.qmask[tid];
+        sSM[tid] = pl.smask[tid];
+    }
+    __syncthreads();
+    const int L = pl.L;
+    const int nlev = L > 3 ? L - 3 : 1;
+    uint64_t *my = stk + ((uint64_t)blockIdx.x * (kBlock / 64) + wv) * (uint64_t)nlev * wmax;
+    uint64_t cnt = 0, hs = 0, hx = 0;
+    auto xget = [&](int q) -> uint32_t { return sX[wv][q]; };
+    for (;;) {
+        unsigned long long it = 0;
+        if (lane == 0) it = atomicAdd(cursor, 1ull);
+        it = __shfl(it, 0, 64);
+        const uint64_t r = g0 + it;
+        if (r >= g1) break;
+        const HubRec h = hubs[hub_of_row[r]];
+        const uint32_t jl = (uint32_t)(r - h.joff);
+        if (pl.skip_split && jl < h.split) continue;
+        if (L < 3) continue;
+        const uint32_t d = h.d, W = (d + 63) >> 6;
+        const uint32_t R = (W + 63) >> 6;
+        const uint64_t *Eh = E + h.eoff;
+        const uint32_t *A = g.ocol + h.rbeg;
+        if (lane == 0) sX[wv][1] = jl;
+        __builtin_amdgcn_wave_barrier();
+
+        auto level_word = [&](int l, uint32_t w) -> uint64_t {
+            uint64_t t = valid_mask(w, d);
+            uint32_t qm = sQM[l];
+            while (qm) {
+                const int q = __builtin_ctz(qm);
+                qm &= qm - 1;
+                t &= Eh[(uint64_t)xget(q) * W + w];
+            }
+            for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
+            uint32_t sm = sSM[l];
+            while (sm) {
+                const int s = __builtin_ctz(sm);
+                sm &= sm - 1;
+                t &= gt_mask(xget(s), w);
+            }
+            return t;
+        };
+
+        if (L == 3) {
+            for (uint32_t rr = 0; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                if (w >= W) break;
+                const uint64_t t = level_word(2, w);
+                cnt += (uint64_t)__popcll(t);
+                if (pl.emit && t) emit_word(out, pl, 2, h.u, A, t, w, xget, hs, hx);
+            }
+            continue;
+        }
+        for (uint32_t rr = 0; rr < R; rr++) {
+            const uint32_t w = rr * 64 + lane;
+            if (w < W) my[w] = level_word(2, w);
+        }
+        int l = 3;
+        uint32_t pos = 0;
+        for (;;) {
+            // next candidate of C_{l-1} at index >= pos
+            uint32_t xi = 0xffffffffu;
+            const uint32_t pw = pos >> 6;
+            for (uint32_t rr = pw >> 6; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                uint64_t v = (w < W) ? my[(uint64_t)(l - 3) * wmax + w] : 0ull;
+                if (w < pw) v = 0ull;
+                else if (w == pw) v &= (~0ull << (pos & 63u));
+                const uint64_t bal = __ballot(v != 0ull);
+                if (bal) {
+                    const int f = __builtin_ctzll(bal);
+                    const uint64_t vf = __shfl(v, f, 64);
+                    xi = ((rr * 64 + (uint32_t)f) << 6) + (uint32_t)__builtin_ctzll(vf);
+                    break;
+                }
+            }
+            if (xi == 0xffffffffu) {
+                if (l == 3) break;
+                --l;
+                pos = xget(l - 1) + 1u;
+                continue;
+            }
+            if (lane == 0) sX[wv][l - 1] = xi;
+            __builtin_amdgcn_wave_barrier();
+            if (l == L - 1) {
+                for (uint32_t rr = 0; rr < R; rr++) {
+                    const uint32_t w = rr * 64 + lane;
+                    if (w >= W) break;
+                    const uint64_t t = level_word(l, w);
+                    cnt += (uint64_t)__popcll(t);
+                    if (pl.emit && t) emit_word(out, pl, l, h.u, A, t, w, xget, hs, hx);
+                }
+                pos = xi + 1u;
+            } else {
+                for (uint32_t rr = 0; rr < R; rr++) {
+                    const uint32_t w = rr * 64 + lane;
+                    if (w < W) my[(uint64_t)(l - 2) * wmax + w] = level_word(l, w);
+                }
+                ++l;
+                pos = 0;
+            }
+        }
+    }
+    flush_acc(out, cnt, hs, hx);
+}
+
+// ------------------------------------------------------------ launchers
+
+template <int P>
+static hipError_t launch_lds_p(int grid, size_t lds, hipStream_t st, const KGraph &g, const KPlan &pl,
+                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
+                               uint32_t b0, uint32_t b1, unsigned int *cursor, const LdsLayout &lay,
+                               const KOut &out)
+{
+    hipLaunchKernelGGL(lds_batch_kernel<P>, dim3(grid), dim3(kBlock), lds, st, g, pl, recs, bf, bc, b0,
+                       b1, cursor, lay, out);
+    return hipGetLastError();
+}
+
+template <int P>
+static int occupancy_lds_p(size_t lds)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lds_batch_kernel<P>, kBlock, lds) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+template <int P>
+static void set_lds_attr(size_t lds)
+{
+    (void)hipFuncSetAttribute((const void *)lds_batch_kernel<P>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+// Dispatch on the DFS group width.
+hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
+                              const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
+                              uint32_t b0, uint32_t b1, unsigned int *cursor, uint32_t ewords,
+                              const KOut &out, int *grid_out)
+{
+    const uint32_t nlev = pl.L > 3 ? (uint32_t)(pl.L - 3) : 1u;
+    LdsLayout lay = make_lds_layout(ewords, nlev);
+    size_t lds = lay.bytes;
+    int occ = 1;
+    switch (lg) {
+#define BEEP_CASE(LG)                                   \
+    case LG:                                            \
+        set_lds_attr<(1 << LG)>(lds);                   \
+        occ = occupancy_lds_p<(1 << LG)>(lds);          \
+        break;
+        BEEP_CASE(0) BEEP_CASE(1) BEEP_CASE(2) BEEP_CASE(3) BEEP_CASE(4) BEEP_CASE(5) BEEP_CASE(6)
+#undef BEEP_CASE
+    default: return hipErrorInvalidValue;
+    }
+    int grid = occ * cus;
+    if ((uint32_t)grid > b1 - b0) grid = (int)(b1 - b0);
+    if (grid < 1) grid = 1;
+    if (grid_out) *grid_out = grid;
+    switch (lg) {
+#define BEEP_CASE(LG) \
+    case LG: return launch_lds_p<(1 << LG)>(grid, lds, st, g, pl, recs, bf, bc, b0, b1, cursor, lay, out);
+        BEEP_CASE(0) BEEP_CASE(1) BEEP_CASE(2) BEEP_CASE(3) BEEP_CASE(4) BEEP_CASE(5) BEEP_CASE(6)
+#undef BEEP_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
+                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals)
+{
+    hipLaunchKernelGGL(hub_hash_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
+                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint32_t *keys,
+                             const uint32_t *vals, uint64_t *E)
+{
+    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, keys,
+                       vals, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
+                          const uint32_t *hor, uint64_t g0, uint64_t g1, unsigned long long *cursor,
+                          const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out)
+{
+    hipLaunchKernelGGL(hub_dfs_kernel, dim3(grid), dim3(kBlock), 0, st, g, pl, hubs, hor, g0, g1, cursor,
+                       E, stk, wmax, out);
+    return hipGetLastError();
+}
+
+int hub_dfs_occupancy()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hub_dfs_kernel, kBlock, 0) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace beep

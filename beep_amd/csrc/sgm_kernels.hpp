@@ -1,0 +1,61 @@
+// sgm_kernels.hpp -- kernel-side data layout shared by the HIP kernels and the
+// host engine.
+#pragma once
+
+#include <cstdint>
+
+namespace beep {
+
+constexpr int kBlock = 256;     // threads per workgroup (4 wave64s)
+constexpr int kMaxLevels = 10;  // query positions (DEPTH)
+
+// Query plan as the kernels consume it (by value).  Level l in [2, L) is
+// the candidate set C_l:
+//   C_l = {i in [0,d) \ {x_1..x_{l-1}} : E[x_q][i] = 1 for q in qmask[l],
+//          i > x_s for s in smask[l]}
+// which is count_tri for l = 2 and compute_intersection for l >= 3
+// (device_utils.cuh:179-209, 426-456).
+struct KPlan {
+    int32_t k;
+    int32_t L;           // k - LUNMAT
+    int32_t skip_split;  // skip x_1 < split(u) (sgm_kernelsLD.cuh:60)
+    int32_t emit;        // BEEP_EMIT_*
+    uint32_t qmask[kMaxLevels];
+    uint32_t smask[kMaxLevels];
+};
+
+struct KGraph {
+    const uint64_t *rowptr;  // n + 1
+    const uint32_t *ocol;    // degree-oriented rows
+    const uint32_t *nlow;    // #neighbours with a smaller key
+    const uint32_t *hlen;    // bitmap-build scan length (starts at nlow if asc, else 0)
+    int32_t asc;
+};
+
+// Per-root record of an LDS batch, offsets relative to the batch.
+struct RootRec {
+    uint64_t rbeg;   // rowptr[u]
+    uint32_t u;      // root vertex
+    uint32_t d;      // degree
+    uint32_t split;  // nlow[u]
+    uint32_t doff;   // offset of the root's row in the batch's concatenated rows
+    uint32_t eoff;   // offset of the root's bitmap (u64 words)
+    uint32_t hoff;   // offset of the root's hash table (slots)
+};
+
+// Per-hub record (bitmap and hash table in HBM).
+struct HubRec {
+    uint64_t rbeg;   // rowptr[u]
+    uint64_t eoff;   // bitmap offset in the arena (u64 words)
+    uint64_t hoff;   // hash offset in the arena (slots)
+    uint32_t u, d, split, hslots;
+    uint64_t joff;   // first global row index of this hub
+};
+
+struct KOut {
+    unsigned long long *acc;  // [0] count [1] hash_sum [2] hash_xor [3] written
+    uint32_t *buf;            // device tuple buffer (cap * L words) or null
+    uint64_t cap;
+};
+
+}  // namespace beep

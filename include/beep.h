@@ -1,0 +1,187 @@
+/*
+ * beep.h -- C ABI of the MI355X-native subgraph-enumeration path.
+ *
+ * The reference (researchgroup-zx93920/BEEP) is a monolithic CUDA executable;
+ * its path sits behind three nested boundaries, each replaced here:
+ *   1. the CLI contract  (src/main.cu:27-344, include/Config.h:336-429)
+ *        -> beep_amd/bin/beep, a thin client of this ABI;
+ *   2. the in-process C++ boundary graph::SG_Match<uint>(task, by, dev, cutoff,
+ *      ndev).run(COOCSRGraph_d&, COOCSRGraph&) (subgraph_matching.cuh:113,178)
+ *        -> beep_plan_* + beep_prepare + beep_count below;
+ *   3. the kernel ABI sgm_kernel_central_node_function<uint,320,16>(GLOBAL_HANDLE,
+ *      queue...) plus __constant__ tables (sgm_kernelsLD.cuh:7-11,
+ *      common_utils.cuh:6-16, host_utils.cuh:12-24)
+ *        -> internal to libbeep.so (beep_amd/csrc/sgm_kernels.hip).
+ *
+ * Conventions: opaque handles with caller-owned *_free; every entry point
+ * returns 0 on success and a negative errno-style code otherwise (never exits);
+ * beep_last_error() describes the last failure of the calling thread.  Handles
+ * are not thread-safe except a beep_server, whose submit/poll are MPSC-safe.
+ * Device memory belongs to the handle that allocated it.
+ */
+#ifndef BEEP_H
+#define BEEP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEEP_OK 0
+#define BEEP_EIO (-5)
+#define BEEP_ENOMEM (-12)
+#define BEEP_EDEVICE (-19)
+#define BEEP_EINVAL (-22)
+#define BEEP_ERANGE (-34)
+#define BEEP_ENOSYS (-38)
+#define BEEP_ETIMEDOUT (-110)
+
+#define BEEP_MODE_SGM 0 /* -m sgm, Config.h:80-81 */
+#define BEEP_MODE_SGC 1 /* -m sgc, Config.h:83-84 */
+
+#define BEEP_MAXK 10 /* DEPTH, host_utils.cuh:10 */
+
+/* Emission of the last materialised level (sgm: full matches). */
+#define BEEP_EMIT_NONE 0   /* count only (the reference's behaviour)          */
+#define BEEP_EMIT_HASH 1   /* + order-independent multiset hash of the tuples */
+#define BEEP_EMIT_BUFFER 2 /* + hash + first match_cap tuples into a buffer   */
+
+typedef struct beep_graph beep_graph;
+typedef struct beep_plan beep_plan;
+typedef struct beep_prep beep_prep;
+typedef struct beep_server beep_server;
+
+/* ------------------------------------------------------------------------ */
+/* Graph loading.  Replaces EdgeListFile::get_edges + CSRCOO::from_edgelist +  */
+/* get_stats (main.cu:70-132; FIleReader.cuh:52-95,101-127,130-212,299-389;    */
+/* CSRCOO.cuh:37-80; utils.cuh:186-261).                                       */
+/* ------------------------------------------------------------------------ */
+
+/* .bel / .tsv / .mtx chosen by extension; sort_edges mirrors -s (main.cu:88-91). */
+int beep_graph_load(const char *path, int sort_edges, beep_graph **out);
+
+/* Edges in CSR-build order (grouped by ascending src); self loops dropped. */
+int beep_graph_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t n_edges,
+                          beep_graph **out);
+
+/* Deterministic synthetic generators (symmetrised, no self loops or
+ * duplicates, rows sorted).  R-MAT draws `samples` edges with quadrant
+ * probabilities (a,b,c, d = 1-a-b-c); ids >= n_keep are rejected when
+ * n_keep < 2^scale.  Chung-Lu draws `samples` endpoint pairs with weights
+ * w_i proportional to (i + i0)^(-1/(gamma-1)). */
+int beep_graph_rmat(uint32_t scale, uint64_t samples, double a, double b, double c,
+                    uint64_t seed, uint32_t n_keep, beep_graph **out);
+int beep_graph_chung_lu(uint32_t n, uint64_t samples, double gamma, double i0, uint64_t seed,
+                        beep_graph **out);
+
+typedef struct {
+    uint32_t n;          /* rows (max src + 1) */
+    uint64_t m;          /* stored directed entries */
+    uint32_t max_degree;
+    uint32_t cutoff;     /* ceil(mean + 6 sd), utils.cuh:260 */
+    double mean, sd;
+    const uint64_t *rowptr; /* host views, valid while the handle lives */
+    const uint32_t *col;
+} beep_graph_desc;
+
+int beep_graph_describe(const beep_graph *g, beep_graph_desc *out);
+/* Writes 24-byte {dst, src, 0} records (FIleReader.cuh:613-637). */
+int beep_graph_write_bel(const beep_graph *g, const char *path);
+void beep_graph_free(beep_graph *g);
+
+/* ------------------------------------------------------------------------ */
+/* Query plan.  Replaces SG_Match::preprocess_query / detect_symmetry /       */
+/* detect_reuse / check_unmat (subgraph_matching.cuh:257-631), with its own   */
+/* automorphism-orbit search in place of nauty's densenauty.                  */
+/* ------------------------------------------------------------------------ */
+
+int beep_plan_from_mtx(const char *path, int mode, beep_plan **out);
+int beep_plan_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t n_edges, int mode,
+                         beep_plan **out);
+
+typedef struct {
+    int k, mode, lunmat, asc, max_qdeg, min_qdeg;
+    uint32_t seq[BEEP_MAXK];
+    uint32_t qdeg[BEEP_MAXK];
+    uint32_t qedge_ptr[BEEP_MAXK + 1];
+    uint32_t qedge[BEEP_MAXK * (BEEP_MAXK - 1) / 2];
+    uint32_t sym_ptr[BEEP_MAXK + 1];
+    uint32_t sym[BEEP_MAXK * (BEEP_MAXK - 1) / 2];
+} beep_plan_desc;
+
+int beep_plan_describe(const beep_plan *p, beep_plan_desc *out);
+void beep_plan_free(beep_plan *p);
+
+/* ------------------------------------------------------------------------ */
+/* Data preparation.  Replaces initialize / peel_data / degree_ordering and  */
+/* the root queue of count_subgraphs (subgraph_matching.cuh:633-764,          */
+/* 1080-1131, 1282-1336; GraphQueue.cuh:85-120).  The graph handle is not    */
+/* mutated (the reference swaps arrays inside COOCSRGraph_d, :746-750).       */
+/* ------------------------------------------------------------------------ */
+
+int beep_prepare(const beep_graph *g, const beep_plan *p, beep_prep **out);
+
+typedef struct {
+    uint32_t n;
+    uint64_t m;              /* peeled entries */
+    uint64_t n_roots;        /* |R| */
+    uint64_t q0_size;        /* |Q0| */
+    uint64_t boundary;       /* #{d > cutoff} in Q0 */
+    uint32_t cutoff, max_degree, max_root_degree;
+    int peel_rounds, asc;
+    uint64_t alg_bytes;      /* B_alg */
+    uint64_t n_hub_roots;    /* roots whose bitmap lives in HBM */
+    uint64_t n_lds_batches;  /* LDS-resident root batches */
+    const uint64_t *rowptr;  /* host views of the prepared graph */
+    const uint32_t *col;     /* peeled, id ascending */
+    const uint32_t *ocol;    /* peeled, degree oriented */
+    const uint32_t *nlow;    /* #neighbours with smaller (degree, id) key */
+    const uint32_t *roots;   /* R in processing order */
+} beep_prep_desc;
+
+int beep_prep_describe(const beep_prep *pp, beep_prep_desc *out);
+/* Copies the prepared graph to HBM on `device` (idempotent). */
+int beep_prep_upload(beep_prep *pp, int device);
+void beep_prep_free(beep_prep *pp);
+
+/* ------------------------------------------------------------------------ */
+/* Enumeration.  Replaces count_subgraphs' launch of                          */
+/* sgm_kernel_central_node_function + final_counter + the Counter print       */
+/* (subgraph_matching.cuh:1138-1277; sgm_kernelsLD.cuh:7-273).                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int device;             /* HIP device ordinal */
+    int emit;               /* BEEP_EMIT_* */
+    uint64_t match_cap;     /* BEEP_EMIT_BUFFER: tuples to keep */
+    uint32_t *match_buf;    /* host buffer of match_cap * (k - lunmat) u32, or NULL */
+    int lanes;              /* 0 = automatic, else lanes per DFS group (1..64) */
+    int donate;             /* 0 = off; 1 = donate level-2 subtrees of hub roots */
+    uint32_t donate_min;    /* donation threshold: min |C_2| of a donated subtree */
+    int timing;             /* 1 = record HIP events around the enumeration */
+} beep_opts;
+
+typedef struct {
+    uint64_t count;         /* "------------- Counter = N" */
+    uint64_t hash_sum, hash_xor;
+    uint64_t n_emitted, n_written;
+    double t_enum_ms;       /* first enumeration launch -> counter readback */
+    double t_hub_ms, t_lds_ms; /* device time of the hub and LDS phases */
+    uint64_t donations;     /* subtrees donated through the ring */
+} beep_result;
+
+void beep_opts_default(beep_opts *o);
+int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_result *out);
+
+/* Tuple hash used by BEEP_EMIT_HASH: sum and xor over emitted tuples. */
+uint64_t beep_tuple_hash(const uint32_t *t, int len);
+
+const char *beep_last_error(void);
+const char *beep_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
