@@ -171,10 +171,13 @@ class Prepared:
     def upload(self, device=0):
         check(lib().beep_prep_upload(self._h, device))
 
-    def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0):
+    def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0):
         o = _ffi.Opts()
         lib().beep_opts_default(C.byref(o))
         o.device, o.emit, o.lanes = device, emit, lanes
+        o.donate = 1 if donate else 0
+        if donate_min:
+            o.donate_min = donate_min
         buf = None
         if emit == EMIT_BUFFER and match_cap:
             buf = np.zeros(match_cap * self.plan.width, dtype=np.uint32)

@@ -20,7 +20,7 @@ struct beep_prep { std::unique_ptr<beep::Prep> pp; };
 namespace beep {
 
 void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res);
-void upload_prep(Prep &pp, int device);
+void upload_prep(Prep &pp, int device, int skip_split);
 
 static thread_local std::string g_last_error;
 
@@ -224,7 +224,7 @@ int beep_prep_upload(beep_prep *pp, int device)
 {
     return guarded([&] {
         if (!pp) throw Error(BEEP_EINVAL, "null argument");
-        beep::upload_prep(*pp->pp, device);
+        beep::upload_prep(*pp->pp, device, pp->pp->skip_split);
     });
 }
 
@@ -237,7 +237,7 @@ void beep_opts_default(beep_opts *o)
     o->device = 0;
     o->emit = BEEP_EMIT_NONE;
     o->lanes = 0;
-    o->donate = 0;
+    o->donate = 1;
     o->donate_min = 64;
     o->timing = 1;
 }

@@ -91,6 +91,10 @@ struct Plan {
     // derived, used by the kernels
     int L = 0;            // last materialised level + 1 = k - lunmat
     int skip_split = 0;   // SYMNODE_PTR[2] == 1 (sgm_kernelsLD.cuh:60)
+    // Every candidate of every level is bound above x_1 (through the chain of
+    // symmetry constraints) and x_1 >= split: the enumeration only touches row
+    // indices >= split, so a root can be reduced to its larger-key neighbours.
+    int above_split = 0;
     uint16_t qmask[kMaxK] = {};  // positions QEDGE(l)[1:]
     uint16_t smask[kMaxK] = {};  // positions s in SYM(l), s > 0
 };
@@ -121,8 +125,13 @@ struct Prep {
     uint64_t q0_size = 0, boundary = 0;
     uint64_t alg_bytes = 0;
 
-    // Tiering: roots[0, n_hub) have their bitmap in HBM, the rest are packed
-    // into LDS batches.  Within each tier roots keep the descending order.
+    // Work list: the roots that carry work, with the part of the oriented row
+    // the enumeration uses (offset, length), ordered by length descending.
+    // work[0, n_hub) keep their bitmap in HBM ("hubs"); the rest are packed
+    // into LDS batches.
+    int reduced = 0;                  // rows start at the split (above_split plans)
+    int skip_split = 0;               // plan's x_1 < split skip
+    std::vector<uint32_t> wu, woff, wlen;
     uint32_t n_hub = 0;
     std::vector<LdsBatch> batches;
     std::vector<uint8_t> batch_lanes_log2;  // per batch: log2 of DFS-group lanes

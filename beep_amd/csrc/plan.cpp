@@ -180,6 +180,17 @@ std::unique_ptr<Plan> plan_from_edges(const uint32_t *src, const uint32_t *dst, 
         p->qmask[l] = qm;
         p->smask[l] = sm;
     }
+    // x_1 >= split when skip_split; level l is bound above x_1 when one of its
+    // symmetry constraints refers to a position that is.
+    bool bound[kMaxK] = {};
+    bound[1] = true;
+    bool all = p->skip_split != 0;
+    for (int l = 2; l < p->L; l++) {
+        for (int s = 1; s < l; s++)
+            if (((p->smask[l] >> s) & 1u) && bound[s]) bound[l] = true;
+        if (!bound[l]) all = false;
+    }
+    p->above_split = all ? 1 : 0;
     return p;
 }
 

@@ -189,6 +189,34 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
         bytes.fetch_add(acc);
     }, 1024);
     pp->alg_bytes = bytes.load();
+    // Work list.  With above_split plans a root keeps only its suffix of
+    // larger keys (asc rows), which drops the skipped x_1 < split items and
+    // every bitmap entry they alone would use; roots left empty carry no work.
+    pp->reduced = p.above_split;
+    pp->skip_split = p.skip_split;
+    {
+        std::vector<uint32_t> idx;
+        idx.reserve(pp->roots.size());
+        for (uint32_t r = 0; r < (uint32_t)pp->roots.size(); r++) {
+            uint32_t u = pp->roots[r];
+            uint32_t len = pp->reduced ? deg[u] - pp->nlow[u] : deg[u];
+            if (len) idx.push_back(r);
+        }
+        auto len_of = [&](uint32_t r) {
+            uint32_t u = pp->roots[r];
+            return pp->reduced ? deg[u] - pp->nlow[u] : deg[u];
+        };
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return len_of(a) > len_of(b); });
+        pp->wu.resize(idx.size());
+        pp->woff.resize(idx.size());
+        pp->wlen.resize(idx.size());
+        for (size_t i = 0; i < idx.size(); i++) {
+            uint32_t u = pp->roots[idx[i]];
+            pp->wu[i] = u;
+            pp->woff[i] = pp->reduced ? pp->nlow[u] : 0u;
+            pp->wlen[i] = len_of(idx[i]);
+        }
+    }
     build_tiers(*pp);
     return pp;
 }
@@ -198,26 +226,25 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
 // bitmap in HBM and are processed by the hub kernels.
 void build_tiers(Prep &pp)
 {
-    const auto &rp = pp.rowptr;
-    auto deg = [&](uint32_t u) { return (uint32_t)(rp[u + 1] - rp[u]); };
+    auto deg = [&](uint32_t i) { return pp.wlen[i]; };
     auto fits_lds = [&](uint32_t d) {
         uint64_t w = (d + 63) / 64;
         return (uint64_t)d * w <= LdsBudget::kMaxEWords && d <= LdsBudget::kMaxSumDeg &&
                hash_slots_for(d) <= LdsBudget::kMaxHash;
     };
     uint32_t nh = 0;
-    while (nh < pp.roots.size() && !fits_lds(deg(pp.roots[nh]))) nh++;
+    while (nh < pp.wlen.size() && !fits_lds(deg(nh))) nh++;
     pp.n_hub = nh;
     pp.batches.clear();
     pp.batch_lanes_log2.clear();
     uint32_t i = nh;
-    const uint32_t nr = (uint32_t)pp.roots.size();
+    const uint32_t nr = (uint32_t)pp.wlen.size();
     while (i < nr) {
         LdsBatch b{i, 0};
         uint64_t sd = 0, sw = 0, sh = 0;
-        const uint32_t w0 = (deg(pp.roots[i]) + 63) / 64;
+        const uint32_t w0 = (deg(i) + 63) / 64;
         while (i < nr && b.count < LdsBudget::kMaxRoots) {
-            uint32_t d = deg(pp.roots[i]);
+            uint32_t d = deg(i);
             uint64_t w = (d + 63) / 64;
             if (b.count && (sd + d > LdsBudget::kMaxSumDeg || sw + d * w > LdsBudget::kMaxEWords ||
                             sh + hash_slots_for(d) > LdsBudget::kMaxHash))

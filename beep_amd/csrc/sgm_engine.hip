@@ -31,8 +31,9 @@ hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const Hu
                              const uint32_t *hor, uint64_t g0, uint64_t g1, const uint32_t *keys,
                              const uint32_t *vals, uint64_t *E);
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
-                          const uint32_t *hor, uint64_t g0, uint64_t g1, unsigned long long *cursor,
-                          const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out);
+                          const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
+                          const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
+                          const HubQueue &hq, int mode);
 int hub_dfs_occupancy();
 
 #define HIP_CHECK(x)                                                                            \
@@ -59,9 +60,13 @@ static T *dev_upload(const std::vector<T> &v)
     return p;
 }
 
+constexpr int kCtrAcc = 0, kCtrHub = 4, kCtrLds = 68, kCtrQTail = 84, kCtrQHead = 85, kCtrSlot = 86,
+              kCtrDonated = 87, kCtrWords = 88;
+
 struct HubRound {
     uint32_t h0, h1;      // hub index range
-    uint64_t g0, g1;      // global row range
+    uint64_t g0, g1;      // global row range (bitmap build)
+    uint64_t i0, i1;      // global item range (DFS)
     uint64_t ewords;      // bitmap words used
     uint64_t hslots;      // hash slots used
 };
@@ -77,7 +82,8 @@ struct DevPrep {
     uint32_t nbatches = 0;
     // hub tier
     HubRec *hubs = nullptr;
-    uint32_t *hub_of_row = nullptr;
+    uint32_t *hub_of_row = nullptr, *hub_of_item = nullptr;
+    int skip_split = -1;  // plan property the hub items were built for
     uint32_t nhubs = 0;
     uint32_t wmax = 0;
     std::vector<HubRound> rounds;
@@ -86,7 +92,12 @@ struct DevPrep {
     uint32_t *keys = nullptr, *vals = nullptr;
     uint64_t *stk = nullptr;
     size_t stk_words = 0;
-    // counters: acc[4] | hub cursors[64] | lds cursors[16]
+    // donation queue of the hub DFS
+    HubChunk *queue = nullptr;
+    uint64_t *slots = nullptr;
+    uint64_t queue_cap = 0, slot_cap = 0;
+    // counters: acc[4] | hub cursors[64] | lds cursors[16] | queue tail, head,
+    // slot tail, donated
     unsigned long long *ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
@@ -97,8 +108,8 @@ struct DevPrep {
         if (hipGetDevice(&cur) != hipSuccess) cur = device;
         if (device >= 0) (void)hipSetDevice(device);
         for (void *p : {(void *)rowptr, (void *)ocol, (void *)nlow, (void *)hlen, (void *)recs, (void *)batch_first,
-                        (void *)batch_count, (void *)hubs, (void *)hub_of_row, (void *)arena,
-                        (void *)keys, (void *)vals, (void *)stk, (void *)ctr})
+                        (void *)batch_count, (void *)hubs, (void *)hub_of_row, (void *)hub_of_item, (void *)arena,
+                        (void *)keys, (void *)vals, (void *)stk, (void *)ctr, (void *)queue, (void *)slots})
             if (p) (void)hipFree(p);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -109,9 +120,9 @@ struct DevPrep {
 
 Prep::~Prep() { delete dev; }
 
-static void upload(Prep &pp, int device)
+static void upload(Prep &pp, int device, int p_skip_split)
 {
-    if (pp.dev && pp.dev->device == device) return;
+    if (pp.dev && pp.dev->device == device && pp.dev->skip_split == p_skip_split) return;
     delete pp.dev;
     pp.dev = nullptr;
     int ndev = 0;
@@ -121,6 +132,7 @@ static void upload(Prep &pp, int device)
     auto *d = new DevPrep();
     pp.dev = d;
     d->device = device;
+    d->skip_split = p_skip_split;
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     d->cus = prop.multiProcessorCount;
@@ -130,7 +142,6 @@ static void upload(Prep &pp, int device)
     d->ocol = dev_upload(pp.ocol);
     d->nlow = dev_upload(pp.nlow);
     d->hlen = dev_upload(pp.hlen);
-    auto deg = [&](uint32_t u) { return (uint32_t)(pp.rowptr[u + 1] - pp.rowptr[u]); };
 
     // LDS batches: per-root records with batch-relative offsets.
     {
@@ -141,12 +152,13 @@ static void upload(Prep &pp, int device)
             bc.push_back(b.count);
             uint32_t doff = 0, eoff = 0, hoff = 0;
             for (uint32_t i = 0; i < b.count; i++) {
-                uint32_t u = pp.roots[b.first + i];
+                const uint32_t wi = b.first + i;
+                const uint32_t u = pp.wu[wi];
                 RootRec r{};
-                r.rbeg = pp.rowptr[u];
+                r.rbeg = pp.rowptr[u] + pp.woff[wi];
                 r.u = u;
-                r.d = deg(u);
-                r.split = pp.nlow[u];
+                r.d = pp.wlen[wi];
+                r.split = pp.reduced ? 0u : pp.nlow[u];
                 r.doff = doff;
                 r.eoff = eoff;
                 r.hoff = hoff;
@@ -165,19 +177,24 @@ static void upload(Prep &pp, int device)
     // Hubs: records, row -> hub map and rounds that fit the arena.
     {
         std::vector<HubRec> hubs;
-        std::vector<uint32_t> hor;
-        uint64_t joff = 0;
+        std::vector<uint32_t> hor, hoi;
+        uint64_t joff = 0, ioff = 0;
         for (uint32_t h = 0; h < pp.n_hub; h++) {
-            uint32_t u = pp.roots[h];
+            const uint32_t u = pp.wu[h];
             HubRec r{};
-            r.rbeg = pp.rowptr[u];
+            r.rbeg = pp.rowptr[u] + pp.woff[h];
             r.u = u;
-            r.d = deg(u);
-            r.split = pp.nlow[u];
+            r.d = pp.wlen[h];
+            r.split = pp.reduced ? 0u : pp.nlow[u];
             r.hslots = hash_slots_for(r.d);
             r.joff = joff;
+            r.ioff = ioff;
+            // x_1 < split items are skipped (sgm_kernelsLD.cuh:60): they are not listed.
+            r.jbeg = (p_skip_split && r.split < r.d) ? r.split : (p_skip_split ? r.d : 0u);
             joff += r.d;
+            ioff += r.d - r.jbeg;
             for (uint32_t j = 0; j < r.d; j++) hor.push_back(h);
+            for (uint32_t j = r.jbeg; j < r.d; j++) hoi.push_back(h);
             d->wmax = std::max<uint32_t>(d->wmax, (r.d + 63) / 64);
             hubs.push_back(r);
         }
@@ -190,7 +207,7 @@ static void upload(Prep &pp, int device)
             uint64_t w = (uint64_t)hubs[h].d * ((hubs[h].d + 63) / 64);
             uint64_t s = hubs[h].hslots;
             if (h > h0 && (ew + w) * 8 + (hsl + s) * 8 > budget_bytes) {
-                d->rounds.push_back({h0, h, hubs[h0].joff, hubs[h].joff, ew, hsl});
+                d->rounds.push_back({h0, h, hubs[h0].joff, hubs[h].joff, hubs[h0].ioff, hubs[h].ioff, ew, hsl});
                 h0 = h;
                 ew = hsl = 0;
             }
@@ -202,13 +219,14 @@ static void upload(Prep &pp, int device)
                 throw Error(BEEP_ENOMEM, "hub root bitmap exceeds the HBM arena budget");
         }
         if (!hubs.empty())
-            d->rounds.push_back({h0, (uint32_t)hubs.size(), hubs[h0].joff, joff, ew, hsl});
+            d->rounds.push_back({h0, (uint32_t)hubs.size(), hubs[h0].joff, joff, hubs[h0].ioff, ioff, ew, hsl});
         for (const auto &r : d->rounds) {
             d->arena_words = std::max(d->arena_words, r.ewords);
             d->arena_slots = std::max(d->arena_slots, r.hslots);
         }
         d->hubs = dev_upload(hubs);
         d->hub_of_row = dev_upload(hor);
+        d->hub_of_item = dev_upload(hoi);
         d->nhubs = (uint32_t)hubs.size();
         if (d->nhubs) {
             d->arena = dev_alloc<uint64_t>(d->arena_words);
@@ -216,7 +234,7 @@ static void upload(Prep &pp, int device)
             d->vals = dev_alloc<uint32_t>(d->arena_slots);
         }
     }
-    d->ctr = dev_alloc<unsigned long long>(4 + 64 + 16);
+    d->ctr = dev_alloc<unsigned long long>(kCtrWords);
 }
 
 static KPlan make_kplan(const Plan &p, int emit)
@@ -235,7 +253,7 @@ static KPlan make_kplan(const Plan &p, int emit)
 
 void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res)
 {
-    upload(pp, o.device);
+    upload(pp, o.device, p.skip_split);
     DevPrep &d = *pp.dev;
     HIP_CHECK(hipSetDevice(d.device));
     if (p.min_qdeg != pp.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != pp.asc)
@@ -243,7 +261,7 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     const int width = p.L;
     uint32_t *dbuf = nullptr;
     if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) dbuf = dev_alloc<uint32_t>(o.match_cap * (uint64_t)width);
-    KOut out{d.ctr, dbuf, (o.emit == BEEP_EMIT_BUFFER) ? o.match_cap : 0};
+    KOut out{d.ctr + kCtrAcc, dbuf, (o.emit == BEEP_EMIT_BUFFER) ? o.match_cap : 0};
     KPlan kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
     KGraph kg{d.rowptr, d.ocol, d.nlow, d.hlen, pp.asc};
     hipStream_t st = d.stream;
@@ -258,7 +276,28 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
             d.stk_words = need;
         }
     }
-    HIP_CHECK(hipMemsetAsync(d.ctr, 0, sizeof(unsigned long long) * (4 + 64 + 16), st));
+    // donation queue (hub items with |C_2| > donate_min), allocated on first use
+    const bool donate = o.donate && d.nhubs && p.L >= 4;
+    if (donate && !d.queue) {
+        d.queue_cap = 1ull << 22;
+        d.slot_cap = 1ull << 26;
+        d.queue = dev_alloc<HubChunk>(d.queue_cap);
+        d.slots = dev_alloc<uint64_t>(d.slot_cap);
+    }
+    HubQueue hq{};
+    if (donate) {
+        hq.q = d.queue;
+        hq.cap = d.queue_cap;
+        hq.slots = d.slots;
+        hq.slot_cap = d.slot_cap;
+        hq.tail = d.ctr + kCtrQTail;
+        hq.head = d.ctr + kCtrQHead;
+        hq.slot_tail = d.ctr + kCtrSlot;
+        hq.donated = d.ctr + kCtrDonated;
+        hq.split_min = o.donate_min ? o.donate_min : 64;
+        hq.chunk = hq.split_min;
+    }
+    HIP_CHECK(hipMemsetAsync(d.ctr, 0, sizeof(unsigned long long) * kCtrWords, st));
     HIP_CHECK(hipEventRecord(d.ev[0], st));
     // hub phase
     for (size_t ri = 0; ri < d.rounds.size(); ri++) {
@@ -271,11 +310,18 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
         int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)d.cus * 16);
         HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys,
                                     d.vals, d.arena));
-        unsigned long long *cur = d.ctr + 4 + (ri % 64);
+        unsigned long long *cur = d.ctr + kCtrHub + (ri % 64);
         if (ri >= 64) HIP_CHECK(hipMemsetAsync(cur, 0, 8, st));
-        int g3 = (int)std::min<uint64_t>((rows + 3) / 4, (uint64_t)hub_grid);
-        HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, kg, kp, d.hubs, d.hub_of_row, r.g0, r.g1, cur, d.arena,
-                                 d.stk, std::max<uint32_t>(d.wmax, 1), out));
+        const uint64_t items = r.i1 - r.i0;
+        if (items) {
+            if (donate && ri > 0) HIP_CHECK(hipMemsetAsync(d.ctr + kCtrQTail, 0, 3 * 8, st));
+            int g3 = (int)std::min<uint64_t>((items + 3) / 4, (uint64_t)hub_grid);
+            HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, kg, kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur,
+                                     d.arena, d.stk, std::max<uint32_t>(d.wmax, 1), out, hq, 0));
+            if (donate)
+                HIP_CHECK(launch_hub_dfs(hub_grid, st, kg, kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur, d.arena,
+                                         d.stk, std::max<uint32_t>(d.wmax, 1), out, hq, 1));
+        }
     }
     HIP_CHECK(hipEventRecord(d.ev[1], st));
     // LDS phase: consecutive batches with the same group width share a launch
@@ -290,15 +336,15 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
         uint32_t e = b + 1;
         while (e < d.nbatches && std::max<int>(pp.batch_lanes_log2[e], lmin) == lg) e++;
         uint32_t ewords = std::min<uint32_t>(LdsBudget::kMaxEWords, LdsBudget::kMaxSumDeg << std::min(lg, 3));
-        unsigned int *cur = (unsigned int *)(d.ctr + 4 + 64) + (cls % 32);
+        unsigned int *cur = (unsigned int *)(d.ctr + kCtrLds) + (cls % 32);
         HIP_CHECK(launch_lds_batches(lg, d.cus, st, kg, kp, d.recs, d.batch_first, d.batch_count, b, e, cur,
                                      ewords, out, nullptr));
         b = e;
         cls++;
-        if (cls % 32 == 0) HIP_CHECK(hipMemsetAsync(d.ctr + 4 + 64, 0, 16 * 8, st));
+        if (cls % 32 == 0) HIP_CHECK(hipMemsetAsync(d.ctr + kCtrLds, 0, 16 * 8, st));
     }
     HIP_CHECK(hipEventRecord(d.ev[2], st));
-    unsigned long long acc[4];
+    unsigned long long acc[kCtrWords];
     HIP_CHECK(hipMemcpyAsync(acc, d.ctr, sizeof(acc), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipEventRecord(d.ev[3], st));
     HIP_CHECK(hipStreamSynchronize(st));
@@ -314,7 +360,7 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     res.t_enum_ms = t_all;
     res.t_hub_ms = t_hub;
     res.t_lds_ms = t_lds;
-    res.donations = 0;
+    res.donations = acc[kCtrDonated];
     if (dbuf) {
         if (o.match_buf && res.n_written)
             HIP_CHECK(hipMemcpy(o.match_buf, dbuf, res.n_written * width * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -322,6 +368,6 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     }
 }
 
-void upload_prep(Prep &pp, int device) { upload(pp, device); }
+void upload_prep(Prep &pp, int device, int skip_split) { upload(pp, device, skip_split); }
 
 }  // namespace beep

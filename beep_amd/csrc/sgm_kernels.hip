@@ -482,15 +482,30 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
     }
 }
 
+// k-th (0-based) set bit of w
+__device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
+{
+    for (uint32_t i = 0; i < k; i++) w &= w - 1;
+    return (uint32_t)__builtin_ctzll(w);
+}
+
 // Hub DFS: one wave per (hub, x_1) item; lane i owns words i, i+64, ...;
 // candidate stack in a per-wave HBM slot of nlev x wmax words.
+//
+// mode 0 walks the items.  An item whose level-2 candidate set holds more
+// than hq.split_min candidates is donated instead of walked: C_2 is copied to
+// a donation slot and cut into chunks of hq.chunk consecutive candidates,
+// each a (hub, x_1, slot, [lo, hi)) record appended to the queue with one
+// atomic.  mode 1 (the next launch on the stream) drains the queue: every
+// wave of the grid takes chunks and walks the x_2 in [lo, hi) of its chunk.
 __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                                                          const HubRec *__restrict__ hubs,
-                                                         const uint32_t *__restrict__ hub_of_row,
-                                                         uint64_t g0, uint64_t g1,
+                                                         const uint32_t *__restrict__ hub_of_item,
+                                                         uint64_t i0, uint64_t i1,
                                                          unsigned long long *cursor,
                                                          const uint64_t *__restrict__ E,
-                                                         uint64_t *stk, uint32_t wmax, KOut out)
+                                                         uint64_t *stk, uint32_t wmax, KOut out,
+                                                         HubQueue hq, int mode)
 {
     __shared__ uint32_t sX[kBlock / 64][kMaxLevels];
     __shared__ uint32_t sQM[kMaxLevels], sSM[kMaxLevels];
@@ -505,59 +520,35 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     uint64_t *my = stk + ((uint64_t)blockIdx.x * (kBlock / 64) + wv) * (uint64_t)nlev * wmax;
     uint64_t cnt = 0, hs = 0, hx = 0;
     auto xget = [&](int q) -> uint32_t { return sX[wv][q]; };
-    for (;;) {
-        unsigned long long it = 0;
-        if (lane == 0) it = atomicAdd(cursor, 1ull);
-        it = __shfl(it, 0, 64);
-        const uint64_t r = g0 + it;
-        if (r >= g1) break;
-        const HubRec h = hubs[hub_of_row[r]];
-        const uint32_t jl = (uint32_t)(r - h.joff);
-        if (pl.skip_split && jl < h.split) continue;
-        if (L < 3) continue;
-        const uint32_t d = h.d, W = (d + 63) >> 6;
-        const uint32_t R = (W + 63) >> 6;
-        const uint64_t *Eh = E + h.eoff;
-        const uint32_t *A = g.ocol + h.rbeg;
-        if (lane == 0) sX[wv][1] = jl;
-        __builtin_amdgcn_wave_barrier();
 
-        auto level_word = [&](int l, uint32_t w) -> uint64_t {
-            uint64_t t = valid_mask(w, d);
-            uint32_t qm = sQM[l];
-            while (qm) {
-                const int q = __builtin_ctz(qm);
-                qm &= qm - 1;
-                t &= Eh[(uint64_t)xget(q) * W + w];
-            }
-            for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
-            uint32_t sm = sSM[l];
-            while (sm) {
-                const int s = __builtin_ctz(sm);
-                sm &= sm - 1;
-                t &= gt_mask(xget(s), w);
-            }
-            return t;
-        };
+    // per-item context
+    const uint64_t *Eh = nullptr;
+    const uint32_t *A = nullptr;
+    uint32_t d = 0, W = 0, R = 0, u = 0;
 
-        if (L == 3) {
-            for (uint32_t rr = 0; rr < R; rr++) {
-                const uint32_t w = rr * 64 + lane;
-                if (w >= W) break;
-                const uint64_t t = level_word(2, w);
-                cnt += (uint64_t)__popcll(t);
-                if (pl.emit && t) emit_word(out, pl, 2, h.u, A, t, w, xget, hs, hx);
-            }
-            continue;
+    auto level_word = [&](int l, uint32_t w) -> uint64_t {
+        uint64_t t = valid_mask(w, d);
+        uint32_t qm = sQM[l];
+        while (qm) {
+            const int q = __builtin_ctz(qm);
+            qm &= qm - 1;
+            t &= Eh[(uint64_t)xget(q) * W + w];
         }
-        for (uint32_t rr = 0; rr < R; rr++) {
-            const uint32_t w = rr * 64 + lane;
-            if (w < W) my[w] = level_word(2, w);
+        for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
+        uint32_t sm = sSM[l];
+        while (sm) {
+            const int s = __builtin_ctz(sm);
+            sm &= sm - 1;
+            t &= gt_mask(xget(s), w);
         }
+        return t;
+    };
+
+    // DFS below level 2 with C_2 in my[0, W): x_2 runs over C_2 n [pos0, hi2).
+    auto dfs3 = [&](uint32_t pos0, uint32_t hi2) {
         int l = 3;
-        uint32_t pos = 0;
+        uint32_t pos = pos0;
         for (;;) {
-            // next candidate of C_{l-1} at index >= pos
             uint32_t xi = 0xffffffffu;
             const uint32_t pw = pos >> 6;
             for (uint32_t rr = pw >> 6; rr < R; rr++) {
@@ -573,6 +564,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                     break;
                 }
             }
+            if (l == 3 && xi != 0xffffffffu && xi >= hi2) xi = 0xffffffffu;
             if (xi == 0xffffffffu) {
                 if (l == 3) break;
                 --l;
@@ -587,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                     if (w >= W) break;
                     const uint64_t t = level_word(l, w);
                     cnt += (uint64_t)__popcll(t);
-                    if (pl.emit && t) emit_word(out, pl, l, h.u, A, t, w, xget, hs, hx);
+                    if (pl.emit && t) emit_word(out, pl, l, u, A, t, w, xget, hs, hx);
                 }
                 pos = xi + 1u;
             } else {
@@ -599,6 +591,119 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 pos = 0;
             }
         }
+    };
+
+    auto bind = [&](const HubRec &h, uint32_t x1) {
+        d = h.d;
+        W = (d + 63) >> 6;
+        R = (W + 63) >> 6;
+        u = h.u;
+        Eh = E + h.eoff;
+        A = g.ocol + h.rbeg;
+        if (lane == 0) sX[wv][1] = x1;
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    if (mode == 1) {
+        const uint64_t ntail = *hq.tail;
+        const uint64_t nq = ntail < hq.cap ? ntail : hq.cap;
+        for (;;) {
+            unsigned long long c = 0;
+            if (lane == 0) c = atomicAdd(hq.head, 1ull);
+            c = __shfl(c, 0, 64);
+            if (c >= nq) break;
+            const HubChunk ch = hq.q[c];
+            if (ch.lo >= ch.hi) continue;
+            bind(hubs[ch.hub], ch.x1);
+            for (uint32_t rr = 0; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                if (w < W) my[w] = hq.slots[ch.slot + w];
+            }
+            dfs3(ch.lo, ch.hi);
+        }
+        flush_acc(out, cnt, hs, hx);
+        return;
+    }
+
+    constexpr uint32_t kClaim = 4;  // items per cursor claim
+    uint64_t claim_end = 0, r = 0;
+    for (;;) {
+        if (r >= claim_end) {
+            unsigned long long it = 0;
+            if (lane == 0) it = atomicAdd(cursor, (unsigned long long)kClaim);
+            it = __shfl(it, 0, 64);
+            r = i0 + it;
+            claim_end = r + kClaim < i1 ? r + kClaim : i1;
+        }
+        if (r >= i1) break;
+        const uint64_t item = r++;
+        const uint32_t hidx = hub_of_item[item];
+        const HubRec h = hubs[hidx];
+        const uint32_t jl = h.jbeg + (uint32_t)(item - h.ioff);
+        if (L < 3) continue;
+        bind(h, jl);
+        if (L == 3) {
+            for (uint32_t rr = 0; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                if (w >= W) break;
+                const uint64_t t = level_word(2, w);
+                cnt += (uint64_t)__popcll(t);
+                if (pl.emit && t) emit_word(out, pl, 2, u, A, t, w, xget, hs, hx);
+            }
+            continue;
+        }
+        uint32_t c2 = 0;
+        for (uint32_t rr = 0; rr < R; rr++) {
+            const uint32_t w = rr * 64 + lane;
+            if (w < W) {
+                const uint64_t t = level_word(2, w);
+                my[w] = t;
+                c2 += (uint32_t)__popcll(t);
+            }
+        }
+        c2 = (uint32_t)wave_sum_u64(c2);
+        if (hq.q && c2 > hq.split_min) {
+            // donate: publish C_2, then cut it into chunks of hq.chunk candidates
+            unsigned long long sb = 0;
+            if (lane == 0) sb = atomicAdd(hq.slot_tail, (unsigned long long)W);
+            sb = __shfl(sb, 0, 64);
+            if (sb + W <= hq.slot_cap) {
+                const uint32_t nc = (c2 + hq.chunk - 1) / hq.chunk;
+                unsigned long long qb = 0;
+                if (lane == 0) qb = atomicAdd(hq.tail, (unsigned long long)nc);
+                qb = __shfl(qb, 0, 64);
+                if (qb + nc <= hq.cap) {
+                    if (lane == 0) atomicAdd(hq.donated, (unsigned long long)nc);
+                    uint32_t carry = 0;
+                    for (uint32_t rr = 0; rr < R; rr++) {
+                        const uint32_t w = rr * 64 + lane;
+                        const uint64_t t = (w < W) ? my[w] : 0ull;
+                        if (w < W) hq.slots[sb + w] = t;
+                        const uint32_t pc = (uint32_t)__popcll(t);
+                        const uint32_t incl = wave_incl_scan(pc) + carry;
+                        const uint32_t excl = incl - pc;
+                        for (uint32_t c = (excl + hq.chunk - 1) / hq.chunk; c * hq.chunk < incl; c++) {
+                            const uint32_t idx = (w << 6) + select_bit(t, c * hq.chunk - excl);
+                            HubChunk *rec = hq.q + qb + c;
+                            rec->hub = hidx;
+                            rec->x1 = jl;
+                            rec->slot = sb;
+                            rec->lo = idx;
+                            if (c > 0) hq.q[qb + c - 1].hi = idx;
+                        }
+                        carry = __shfl(incl, 63, 64);
+                    }
+                    if (lane == 0) hq.q[qb + nc - 1].hi = W << 6;
+                    continue;
+                }
+                // queue full: neutralise the reserved records that exist, walk inline
+                for (uint64_t c = qb + lane; c < qb + nc && c < hq.cap; c += 64) {
+                    hq.q[c].lo = 0;
+                    hq.q[c].hi = 0;
+                }
+            }
+        }
+        dfs3(0, 0xffffffffu);
     }
     flush_acc(out, cnt, hs, hx);
 }
@@ -682,11 +787,12 @@ hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const Hu
 }
 
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
-                          const uint32_t *hor, uint64_t g0, uint64_t g1, unsigned long long *cursor,
-                          const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out)
+                          const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
+                          const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
+                          const HubQueue &hq, int mode)
 {
-    hipLaunchKernelGGL(hub_dfs_kernel, dim3(grid), dim3(kBlock), 0, st, g, pl, hubs, hor, g0, g1, cursor,
-                       E, stk, wmax, out);
+    hipLaunchKernelGGL(hub_dfs_kernel, dim3(grid), dim3(kBlock), 0, st, g, pl, hubs, hoi, i0, i1, cursor,
+                       E, stk, wmax, out, hq, mode);
     return hipGetLastError();
 }
 

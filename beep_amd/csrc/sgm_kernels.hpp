@@ -49,7 +49,32 @@ struct HubRec {
     uint64_t eoff;   // bitmap offset in the arena (u64 words)
     uint64_t hoff;   // hash offset in the arena (slots)
     uint32_t u, d, split, hslots;
-    uint64_t joff;   // first global row index of this hub
+    uint64_t joff;   // first global row index of this hub (bitmap build)
+    uint64_t ioff;   // first global DFS item of this hub
+    uint32_t jbeg;   // first x_1 of the hub's items
+    uint32_t pad;
+};
+
+// Donated level-2 subtree of a hub item: x_2 in C_2 n [lo, hi), with C_2
+// published at slots[slot, slot + W).
+struct HubChunk {
+    uint32_t hub;
+    uint32_t x1;
+    uint64_t slot;
+    uint32_t lo, hi;
+};
+
+struct HubQueue {
+    HubChunk *q;                    // null: donation off
+    unsigned long long *tail;       // records reserved
+    unsigned long long *head;       // records taken by the draining pass
+    uint64_t cap;
+    uint64_t *slots;                // published C_2 bitmaps
+    unsigned long long *slot_tail;  // words reserved
+    uint64_t slot_cap;
+    unsigned long long *donated;    // records published over the whole call
+    uint32_t split_min;             // donate when |C_2| > split_min
+    uint32_t chunk;                 // candidates per record
 };
 
 struct KOut {
