@@ -38,7 +38,31 @@ def parse():
     ap.add_argument("--workload", default="chunglu1m")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--once", action="store_true",
+                    help="profiling mode: one launch sequence per query, no warmup, no CPU leg")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device (counter calibration)")
     return ap.parse_args()
+
+
+def calib_copy(mib):
+    """Device-to-device copy of a known size through the HIP runtime (the
+    runtime's copy kernel), used to scale FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"calibration copy: {n} bytes read, {n} bytes written", file=sys.stderr)
 
 
 def read_traffic(workload):
@@ -83,6 +107,15 @@ def main():
     for _, _, _, pp in preps:
         pp.upload(0)
     t_upload = time.time() - t0
+
+    if args.once:
+        for name, task, plan, pp in preps:
+            r = pp.count(emit=B.EMIT_NONE, lanes=args.lanes)
+            print(json.dumps({"query": name, "count": r["count"], "t_enum_ms": r["t_enum_ms"],
+                              "alg_bytes": pp.alg_bytes, "donations": r["donations"]}))
+        if args.calib_copy:
+            calib_copy(args.calib_copy)
+        return
 
     # correctness run with tuple hashes (untimed)
     gpu_ref = []
