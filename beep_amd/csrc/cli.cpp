@@ -46,7 +46,7 @@ int main(int argc, char **argv)
     int mode = -1;
     const char *task = "kc";
     int opt;
-    while ((opt = getopt(argc, argv, "g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:c:n:")) >= 0) {
+    while ((opt = getopt(argc, argv, ":g:r:d:m:x:o:a:k:h:v:s:p:e:q:w:t:c:n:")) >= 0) {
         switch (opt) {
         case 'g': gpath = optarg; break;
         case 't': tpath = optarg; break;
@@ -60,19 +60,25 @@ int main(int argc, char **argv)
                      !strcmp(optarg, "cd"))
                 mode = -1;
             else {
+                // the reference exits with status 0 here (Config.h:86-87)
                 fprintf(stderr, "Unrecognized -mt option (Main TASK): %s\n", optarg);
-                return 2;
+                return 0;
             }
             break;
         case 's': sort_edges = 1; break;
         case 'n': ndev = atoi(optarg); break;
         case 'h': usage(); return 0;
-        case 'r': case 'x': case 'o': case 'a': case 'v': case 'k': case 'p': case 'e': case 'q':
-        case 'w': case 'c': break;
-        default:
+        case ':':
+            if (optopt == 'h') { usage(); return 0; }
             fprintf(stderr, "\nUnrecognized option!\n");
             usage();
-            return 2;
+            return 0;
+        case 'r': case 'x': case 'o': case 'a': case 'v': case 'k': case 'p': case 'e': case 'q':
+        case 'w': case 'c': break;
+        default:  // the reference prints usage and exits with status 0 (Config.h:421-424)
+            fprintf(stderr, "\nUnrecognized option!\n");
+            usage();
+            return 0;
         }
     }
     if (ndev != 1) {

@@ -95,6 +95,8 @@ struct Plan {
     // symmetry constraints) and x_1 >= split: the enumeration only touches row
     // indices >= split, so a root can be reduced to its larger-key neighbours.
     int above_split = 0;
+    // Only bitmap rows >= split are ever read (rows of QEDGE positions).
+    int rows_above_split = 0;
     uint16_t qmask[kMaxK] = {};  // positions QEDGE(l)[1:]
     uint16_t smask[kMaxK] = {};  // positions s in SYM(l), s > 0
 };
@@ -131,6 +133,7 @@ struct Prep {
     // into LDS batches.
     int reduced = 0;                  // rows start at the split (above_split plans)
     int skip_split = 0;               // plan's x_1 < split skip
+    int rows_above_split = 0;         // hub bitmaps keep rows >= split only
     std::vector<uint32_t> wu, woff, wlen;
     uint32_t n_hub = 0;
     std::vector<LdsBatch> batches;

@@ -191,6 +191,14 @@ std::unique_ptr<Plan> plan_from_edges(const uint32_t *src, const uint32_t *dst, 
         if (!bound[l]) all = false;
     }
     p->above_split = all ? 1 : 0;
+    // Bitmap rows are read only for positions that appear in some QEDGE(l)[1:];
+    // when all of them are bound above x_1 >= split, rows below the split are
+    // never read.
+    bool rows = p->skip_split != 0;
+    for (int l = 2; l < p->L; l++)
+        for (int q = 1; q < l; q++)
+            if (((p->qmask[l] >> q) & 1u) && !bound[q]) rows = false;
+    p->rows_above_split = rows ? 1 : 0;
     return p;
 }
 

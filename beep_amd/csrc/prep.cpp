@@ -194,6 +194,7 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
     // every bitmap entry they alone would use; roots left empty carry no work.
     pp->reduced = p.above_split;
     pp->skip_split = p.skip_split;
+    pp->rows_above_split = p.rows_above_split;
     {
         std::vector<uint32_t> idx;
         idx.reserve(pp->roots.size());
@@ -237,17 +238,29 @@ void build_tiers(Prep &pp)
     pp.n_hub = nh;
     pp.batches.clear();
     pp.batch_lanes_log2.clear();
-    uint32_t i = nh;
+    // Batches of one lane class should outnumber the workgroups the GPU runs
+    // at once, so the per-batch degree sum is capped at ~total/1024 per class.
     const uint32_t nr = (uint32_t)pp.wlen.size();
+    auto lg_of = [&](uint32_t d) {
+        uint32_t w = (d + 63) / 64, lg = 0;
+        while ((1u << lg) < w) lg++;
+        return lg;
+    };
+    uint64_t class_sum[8] = {};
+    for (uint32_t j = nh; j < nr; j++) class_sum[lg_of(deg(j))] += deg(j);
+    uint32_t i = nh;
     while (i < nr) {
         LdsBatch b{i, 0};
         uint64_t sd = 0, sw = 0, sh = 0;
-        const uint32_t w0 = (deg(i) + 63) / 64;
+        const uint32_t lg = lg_of(deg(i));
+        uint64_t cap = class_sum[lg] / 1024;
+        if (cap < 64) cap = 64;
+        if (cap > LdsBudget::kMaxSumDeg) cap = LdsBudget::kMaxSumDeg;
         while (i < nr && b.count < LdsBudget::kMaxRoots) {
             uint32_t d = deg(i);
             uint64_t w = (d + 63) / 64;
-            if (b.count && (sd + d > LdsBudget::kMaxSumDeg || sw + d * w > LdsBudget::kMaxEWords ||
-                            sh + hash_slots_for(d) > LdsBudget::kMaxHash))
+            if (b.count && (sd + d > cap || sw + d * w > LdsBudget::kMaxEWords ||
+                            sh + hash_slots_for(d) > LdsBudget::kMaxHash || lg_of(d) != lg))
                 break;
             sd += d;
             sw += d * w;
@@ -255,10 +268,8 @@ void build_tiers(Prep &pp)
             b.count++;
             i++;
         }
-        uint8_t lg = 0;
-        while ((1u << lg) < w0) lg++;
         pp.batches.push_back(b);
-        pp.batch_lanes_log2.push_back(lg);
+        pp.batch_lanes_log2.push_back((uint8_t)lg);
     }
 }
 

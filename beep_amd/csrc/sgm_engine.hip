@@ -34,7 +34,7 @@ hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
                           const HubQueue &hq, int mode);
-int hub_dfs_occupancy();
+int hub_dfs_occupancy(int nlev, uint32_t wmax);
 
 #define HIP_CHECK(x)                                                                            \
     do {                                                                                        \
@@ -74,8 +74,8 @@ struct HubRound {
 struct DevPrep {
     int device = -1;
     int cus = 0;
-    uint64_t *rowptr = nullptr;
-    uint32_t *ocol = nullptr, *nlow = nullptr, *hlen = nullptr;
+    uint32_t *ocol = nullptr;
+    VInfo *vinfo = nullptr;
     // LDS tier
     RootRec *recs = nullptr;
     uint32_t *batch_first = nullptr, *batch_count = nullptr;
@@ -100,19 +100,26 @@ struct DevPrep {
     // slot tail, donated
     unsigned long long *ctr = nullptr;
     hipStream_t stream = nullptr;
+    static constexpr int kSide = 3;      // streams of the LDS lane classes
+    hipStream_t side[kSide] = {};
     hipEvent_t ev[4] = {};
+    hipEvent_t side_done[kSide] = {};
 
     ~DevPrep()
     {
         int cur = 0;
         if (hipGetDevice(&cur) != hipSuccess) cur = device;
         if (device >= 0) (void)hipSetDevice(device);
-        for (void *p : {(void *)rowptr, (void *)ocol, (void *)nlow, (void *)hlen, (void *)recs, (void *)batch_first,
+        for (void *p : {(void *)ocol, (void *)vinfo, (void *)recs, (void *)batch_first,
                         (void *)batch_count, (void *)hubs, (void *)hub_of_row, (void *)hub_of_item, (void *)arena,
                         (void *)keys, (void *)vals, (void *)stk, (void *)ctr, (void *)queue, (void *)slots})
             if (p) (void)hipFree(p);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto &e : side_done)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &s : side)
+            if (s) (void)hipStreamDestroy(s);
         if (stream) (void)hipStreamDestroy(stream);
         if (device >= 0) (void)hipSetDevice(cur);
     }
@@ -137,11 +144,21 @@ static void upload(Prep &pp, int device, int p_skip_split)
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     d->cus = prop.multiProcessorCount;
     HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    for (auto &s : d->side) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     for (auto &e : d->ev) HIP_CHECK(hipEventCreate(&e));
-    d->rowptr = dev_upload(pp.rowptr);
+    for (auto &e : d->side_done) HIP_CHECK(hipEventCreate(&e));
     d->ocol = dev_upload(pp.ocol);
-    d->nlow = dev_upload(pp.nlow);
-    d->hlen = dev_upload(pp.hlen);
+    {
+        std::vector<VInfo> vi(pp.n);
+        parallel_for(pp.n, [&](size_t b, size_t e2) {
+            for (size_t v = b; v < e2; v++) {
+                vi[v].hbeg = pp.rowptr[v] + (pp.asc ? pp.nlow[v] : 0u);
+                vi[v].hlen = pp.hlen[v];
+                vi[v].deg = (uint32_t)(pp.rowptr[v + 1] - pp.rowptr[v]);
+            }
+        });
+        d->vinfo = dev_upload(vi);
+    }
 
     // LDS batches: per-root records with batch-relative offsets.
     {
@@ -191,6 +208,7 @@ static void upload(Prep &pp, int device, int p_skip_split)
             r.ioff = ioff;
             // x_1 < split items are skipped (sgm_kernelsLD.cuh:60): they are not listed.
             r.jbeg = (p_skip_split && r.split < r.d) ? r.split : (p_skip_split ? r.d : 0u);
+            r.rlo = pp.rows_above_split ? std::min(r.split, r.d) : 0u;
             joff += r.d;
             ioff += r.d - r.jbeg;
             for (uint32_t j = 0; j < r.d; j++) hor.push_back(h);
@@ -204,7 +222,7 @@ static void upload(Prep &pp, int device, int p_skip_split)
         uint64_t ew = 0, hsl = 0;
         uint32_t h0 = 0;
         for (uint32_t h = 0; h < (uint32_t)hubs.size(); h++) {
-            uint64_t w = (uint64_t)hubs[h].d * ((hubs[h].d + 63) / 64);
+            uint64_t w = (uint64_t)(hubs[h].d - hubs[h].rlo) * ((hubs[h].d + 63) / 64);
             uint64_t s = hubs[h].hslots;
             if (h > h0 && (ew + w) * 8 + (hsl + s) * 8 > budget_bytes) {
                 d->rounds.push_back({h0, h, hubs[h0].joff, hubs[h].joff, hubs[h0].ioff, hubs[h].ioff, ew, hsl});
@@ -263,13 +281,14 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) dbuf = dev_alloc<uint32_t>(o.match_cap * (uint64_t)width);
     KOut out{d.ctr + kCtrAcc, dbuf, (o.emit == BEEP_EMIT_BUFFER) ? o.match_cap : 0};
     KPlan kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
-    KGraph kg{d.rowptr, d.ocol, d.nlow, d.hlen, pp.asc};
+    KGraph kg{d.ocol, d.vinfo};
     hipStream_t st = d.stream;
     const int nlev = p.L > 3 ? p.L - 3 : 1;
     // hub DFS stack slots
-    int hub_grid = hub_dfs_occupancy() * d.cus;
+    int hub_grid = hub_dfs_occupancy(nlev, std::max<uint32_t>(d.wmax, 1)) * d.cus;
     if (d.nhubs) {
-        size_t need = (size_t)hub_grid * (kBlock / 64) * nlev * std::max<uint32_t>(d.wmax, 1);
+        // per wave: nlev stack rows + the row of x_1 (used when they do not fit LDS)
+        size_t need = (size_t)hub_grid * (kBlock / 64) * (nlev + 1) * std::max<uint32_t>(d.wmax, 1);
         if (need > d.stk_words) {
             if (d.stk) HIP_CHECK(hipFree(d.stk));
             d.stk = dev_alloc<uint64_t>(need);
@@ -299,6 +318,29 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     }
     HIP_CHECK(hipMemsetAsync(d.ctr, 0, sizeof(unsigned long long) * kCtrWords, st));
     HIP_CHECK(hipEventRecord(d.ev[0], st));
+    // The hub rounds run on the main stream; the LDS lane classes are
+    // independent of them and of each other and run on the side streams.
+    for (auto &s : d.side) HIP_CHECK(hipStreamWaitEvent(s, d.ev[0], 0));
+    // LDS phase: consecutive batches with the same group width share a launch
+    int lmin = 0;
+    if (o.lanes > 0) {
+        while ((1 << lmin) < o.lanes && lmin < 6) lmin++;
+    }
+    uint32_t b = 0;
+    int cls = 0;
+    while (b < d.nbatches) {
+        int lg = std::max<int>(pp.batch_lanes_log2[b], lmin);
+        uint32_t e = b + 1;
+        while (e < d.nbatches && std::max<int>(pp.batch_lanes_log2[e], lmin) == lg) e++;
+        uint32_t ewords = std::min<uint32_t>(LdsBudget::kMaxEWords, LdsBudget::kMaxSumDeg << std::min(lg, 3));
+        if (cls >= 32) throw Error(BEEP_ERANGE, "too many lane classes");
+        unsigned int *cur = (unsigned int *)(d.ctr + kCtrLds) + cls;
+        HIP_CHECK(launch_lds_batches(lg, d.cus, d.side[cls % DevPrep::kSide], kg, kp, d.recs, d.batch_first,
+                                     d.batch_count, b, e, cur, ewords, out, nullptr));
+        b = e;
+        cls++;
+    }
+    for (int i = 0; i < DevPrep::kSide; i++) HIP_CHECK(hipEventRecord(d.side_done[i], d.side[i]));
     // hub phase
     for (size_t ri = 0; ri < d.rounds.size(); ri++) {
         const HubRound &r = d.rounds[ri];
@@ -324,25 +366,7 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
         }
     }
     HIP_CHECK(hipEventRecord(d.ev[1], st));
-    // LDS phase: consecutive batches with the same group width share a launch
-    int lmin = 0;
-    if (o.lanes > 0) {
-        while ((1 << lmin) < o.lanes && lmin < 6) lmin++;
-    }
-    uint32_t b = 0;
-    int cls = 0;
-    while (b < d.nbatches) {
-        int lg = std::max<int>(pp.batch_lanes_log2[b], lmin);
-        uint32_t e = b + 1;
-        while (e < d.nbatches && std::max<int>(pp.batch_lanes_log2[e], lmin) == lg) e++;
-        uint32_t ewords = std::min<uint32_t>(LdsBudget::kMaxEWords, LdsBudget::kMaxSumDeg << std::min(lg, 3));
-        unsigned int *cur = (unsigned int *)(d.ctr + kCtrLds) + (cls % 32);
-        HIP_CHECK(launch_lds_batches(lg, d.cus, st, kg, kp, d.recs, d.batch_first, d.batch_count, b, e, cur,
-                                     ewords, out, nullptr));
-        b = e;
-        cls++;
-        if (cls % 32 == 0) HIP_CHECK(hipMemsetAsync(d.ctr + kCtrLds, 0, 16 * 8, st));
-    }
+    for (int i = 0; i < DevPrep::kSide; i++) HIP_CHECK(hipStreamWaitEvent(st, d.side_done[i], 0));
     HIP_CHECK(hipEventRecord(d.ev[2], st));
     unsigned long long acc[kCtrWords];
     HIP_CHECK(hipMemcpyAsync(acc, d.ctr, sizeof(acc), hipMemcpyDeviceToHost, st));
@@ -351,7 +375,11 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     float t_all = 0, t_hub = 0, t_lds = 0;
     HIP_CHECK(hipEventElapsedTime(&t_all, d.ev[0], d.ev[3]));
     HIP_CHECK(hipEventElapsedTime(&t_hub, d.ev[0], d.ev[1]));
-    HIP_CHECK(hipEventElapsedTime(&t_lds, d.ev[1], d.ev[2]));
+    for (int i = 0; i < DevPrep::kSide; i++) {
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, d.ev[0], d.side_done[i]));
+        t_lds = std::max(t_lds, t);
+    }
     res.count = acc[0];
     res.hash_sum = acc[1];
     res.hash_xor = acc[2];

@@ -283,8 +283,9 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
                 }
                 s = (s + 1) & (H - 1);
             }
-            sHBeg[j] = g.rowptr[v] + (g.asc ? g.nlow[v] : 0u);
-            sPre[j] = g.hlen[v];
+            const VInfo vi = g.vinfo[v];
+            sHBeg[j] = vi.hbeg;
+            sPre[j] = vi.hlen;
         }
         __syncthreads();
         block_excl_scan(sPre, sumd, sMisc + 28);
@@ -443,8 +444,9 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
             const HubRec h = hubs[hi];
             const uint32_t jl = (uint32_t)(r - h.joff);
             const uint32_t v = g.ocol[h.rbeg + jl];
-            sBeg[tid] = g.rowptr[v] + (g.asc ? g.nlow[v] : 0u);
-            len = g.hlen[v];
+            const VInfo vi = g.vinfo[v];
+            sBeg[tid] = vi.hbeg;
+            len = vi.hlen;
             sHub[tid] = hi;
             sJl[tid] = jl;
         }
@@ -472,8 +474,12 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                 if (key == 0u) break;
                 if (key == wv2 + 1u) {
                     const uint32_t i = vals[h.hoff + s];
-                    atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)jl * W + (i >> 6)], 1ull << (i & 63));
-                    atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)i * W + (jl >> 6)], 1ull << (jl & 63));
+                    if (jl >= h.rlo)
+                        atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(jl - h.rlo) * W + (i >> 6)],
+                                 1ull << (i & 63));
+                    if (i >= h.rlo)
+                        atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(i - h.rlo) * W + (jl >> 6)],
+                                 1ull << (jl & 63));
                 }
                 s = (s + 1) & (h.hslots - 1);
             }
@@ -489,8 +495,11 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
     return (uint32_t)__builtin_ctzll(w);
 }
 
-// Hub DFS: one wave per (hub, x_1) item; lane i owns words i, i+64, ...;
-// candidate stack in a per-wave HBM slot of nlev x wmax words.
+// Hub DFS: one wave per (hub, x_1) item; lane i owns words i, i+64, ...
+// The candidate stack (nlev x wmax words per wave) and, when the plan reads
+// it, the bitmap row of x_1 live in LDS (LDSBUF) or, for very wide hubs, in a
+// per-wave HBM slot.  Words below the largest symmetry lower bound of a level
+// are neither computed nor scanned.
 //
 // mode 0 walks the items.  An item whose level-2 candidate set holds more
 // than hq.split_min candidates is donated instead of walked: C_2 is copied to
@@ -498,6 +507,7 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 // each a (hub, x_1, slot, [lo, hi)) record appended to the queue with one
 // atomic.  mode 1 (the next launch on the stream) drains the queue: every
 // wave of the grid takes chunks and walks the x_2 in [lo, hi) of its chunk.
+template <bool LDSBUF>
 __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                                                          const HubRec *__restrict__ hubs,
                                                          const uint32_t *__restrict__ hub_of_item,
@@ -507,7 +517,9 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                                                          uint64_t *stk, uint32_t wmax, KOut out,
                                                          HubQueue hq, int mode)
 {
+    extern __shared__ __align__(16) uint64_t hsm[];
     __shared__ uint32_t sX[kBlock / 64][kMaxLevels];
+    __shared__ uint32_t sWlo[kBlock / 64][kMaxLevels];
     __shared__ uint32_t sQM[kMaxLevels], sSM[kMaxLevels];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (tid < kMaxLevels) {
@@ -517,22 +529,41 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     __syncthreads();
     const int L = pl.L;
     const int nlev = L > 3 ? L - 3 : 1;
-    uint64_t *my = stk + ((uint64_t)blockIdx.x * (kBlock / 64) + wv) * (uint64_t)nlev * wmax;
+    // x_1's bitmap row is cached when some level intersects with it
+    bool row1 = false;
+    for (int l = 2; l < L; l++) row1 |= (sQM[l] & 2u) != 0;
+    uint64_t *my;
+    if (LDSBUF) my = hsm + (uint64_t)wv * (nlev + 1) * wmax;
+    else my = stk + ((uint64_t)blockIdx.x * (kBlock / 64) + wv) * (uint64_t)(nlev + 1) * wmax;
+    uint64_t *e1 = my + (uint64_t)nlev * wmax;
     uint64_t cnt = 0, hs = 0, hx = 0;
     auto xget = [&](int q) -> uint32_t { return sX[wv][q]; };
 
     // per-item context
     const uint64_t *Eh = nullptr;
     const uint32_t *A = nullptr;
-    uint32_t d = 0, W = 0, R = 0, u = 0;
+    uint32_t d = 0, W = 0, R = 0, u = 0, rlo = 0;
 
+    // first word of level l that can hold a candidate
+    auto word_lo = [&](int l) -> uint32_t {
+        uint32_t lo = 0;
+        uint32_t sm = sSM[l];
+        while (sm) {
+            const int s = __builtin_ctz(sm);
+            sm &= sm - 1;
+            const uint32_t v = xget(s) >> 6;
+            lo = v > lo ? v : lo;
+        }
+        return lo;
+    };
     auto level_word = [&](int l, uint32_t w) -> uint64_t {
         uint64_t t = valid_mask(w, d);
         uint32_t qm = sQM[l];
         while (qm) {
             const int q = __builtin_ctz(qm);
             qm &= qm - 1;
-            t &= Eh[(uint64_t)xget(q) * W + w];
+            if (q == 1 && row1) t &= e1[w];
+            else t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
         }
         for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
         uint32_t sm = sSM[l];
@@ -550,12 +581,13 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         uint32_t pos = pos0;
         for (;;) {
             uint32_t xi = 0xffffffffu;
-            const uint32_t pw = pos >> 6;
-            for (uint32_t rr = pw >> 6; rr < R; rr++) {
+            uint32_t pw = pos >> 6;
+            const uint32_t wl = sWlo[wv][l - 1];
+            const uint32_t start = pw > wl ? pw : wl;
+            for (uint32_t rr = start >> 6; rr < R; rr++) {
                 const uint32_t w = rr * 64 + lane;
-                uint64_t v = (w < W) ? my[(uint64_t)(l - 3) * wmax + w] : 0ull;
-                if (w < pw) v = 0ull;
-                else if (w == pw) v &= (~0ull << (pos & 63u));
+                uint64_t v = (w < W && w >= start) ? my[(uint64_t)(l - 3) * wmax + w] : 0ull;
+                if (w == pw) v &= (~0ull << (pos & 63u));
                 const uint64_t bal = __ballot(v != 0ull);
                 if (bal) {
                     const int f = __builtin_ctzll(bal);
@@ -573,20 +605,24 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             }
             if (lane == 0) sX[wv][l - 1] = xi;
             __builtin_amdgcn_wave_barrier();
+            const uint32_t lo = word_lo(l);
             if (l == L - 1) {
-                for (uint32_t rr = 0; rr < R; rr++) {
+                for (uint32_t rr = lo >> 6; rr < R; rr++) {
                     const uint32_t w = rr * 64 + lane;
                     if (w >= W) break;
+                    if (w < lo) continue;
                     const uint64_t t = level_word(l, w);
                     cnt += (uint64_t)__popcll(t);
                     if (pl.emit && t) emit_word(out, pl, l, u, A, t, w, xget, hs, hx);
                 }
                 pos = xi + 1u;
             } else {
-                for (uint32_t rr = 0; rr < R; rr++) {
+                for (uint32_t rr = lo >> 6; rr < R; rr++) {
                     const uint32_t w = rr * 64 + lane;
-                    if (w < W) my[(uint64_t)(l - 2) * wmax + w] = level_word(l, w);
+                    if (w < W && w >= lo) my[(uint64_t)(l - 2) * wmax + w] = level_word(l, w);
                 }
+                if (lane == 0) sWlo[wv][l] = lo;
+                __builtin_amdgcn_wave_barrier();
                 ++l;
                 pos = 0;
             }
@@ -598,10 +634,19 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         W = (d + 63) >> 6;
         R = (W + 63) >> 6;
         u = h.u;
+        rlo = h.rlo;
         Eh = E + h.eoff;
         A = g.ocol + h.rbeg;
-        if (lane == 0) sX[wv][1] = x1;
+        if (lane == 0) {
+            sX[wv][1] = x1;
+            sWlo[wv][2] = 0;
+        }
         __builtin_amdgcn_wave_barrier();
+        if (row1)
+            for (uint32_t rr = 0; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                if (w < W) e1[w] = Eh[(uint64_t)(x1 - rlo) * W + w];
+            }
     };
 
     if (mode == 1) {
@@ -642,10 +687,12 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         const uint32_t jl = h.jbeg + (uint32_t)(item - h.ioff);
         if (L < 3) continue;
         bind(h, jl);
+        const uint32_t lo2 = word_lo(2);
         if (L == 3) {
-            for (uint32_t rr = 0; rr < R; rr++) {
+            for (uint32_t rr = lo2 >> 6; rr < R; rr++) {
                 const uint32_t w = rr * 64 + lane;
                 if (w >= W) break;
+                if (w < lo2) continue;
                 const uint64_t t = level_word(2, w);
                 cnt += (uint64_t)__popcll(t);
                 if (pl.emit && t) emit_word(out, pl, 2, u, A, t, w, xget, hs, hx);
@@ -656,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         for (uint32_t rr = 0; rr < R; rr++) {
             const uint32_t w = rr * 64 + lane;
             if (w < W) {
-                const uint64_t t = level_word(2, w);
+                const uint64_t t = w >= lo2 ? level_word(2, w) : 0ull;
                 my[w] = t;
                 c2 += (uint32_t)__popcll(t);
             }
@@ -786,20 +833,41 @@ hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const Hu
     return hipGetLastError();
 }
 
+// Dynamic LDS of the hub DFS: per wave nlev + 1 rows of wmax words.
+static size_t hub_lds_bytes(int nlev, uint32_t wmax) { return (size_t)(kBlock / 64) * (nlev + 1) * wmax * 8; }
+constexpr size_t kHubLdsMax = 96 * 1024;
+
+bool hub_dfs_in_lds(int nlev, uint32_t wmax) { return hub_lds_bytes(nlev, wmax) <= kHubLdsMax; }
+
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
                           const HubQueue &hq, int mode)
 {
-    hipLaunchKernelGGL(hub_dfs_kernel, dim3(grid), dim3(kBlock), 0, st, g, pl, hubs, hoi, i0, i1, cursor,
-                       E, stk, wmax, out, hq, mode);
+    const int nlev = pl.L > 3 ? pl.L - 3 : 1;
+    if (hub_dfs_in_lds(nlev, wmax)) {
+        const size_t lds = hub_lds_bytes(nlev, wmax);
+        (void)hipFuncSetAttribute((const void *)hub_dfs_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(hub_dfs_kernel<true>, dim3(grid), dim3(kBlock), lds, st, g, pl, hubs, hoi, i0, i1,
+                           cursor, E, stk, wmax, out, hq, mode);
+    } else {
+        hipLaunchKernelGGL(hub_dfs_kernel<false>, dim3(grid), dim3(kBlock), 0, st, g, pl, hubs, hoi, i0, i1,
+                           cursor, E, stk, wmax, out, hq, mode);
+    }
     return hipGetLastError();
 }
 
-int hub_dfs_occupancy()
+int hub_dfs_occupancy(int nlev, uint32_t wmax)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hub_dfs_kernel, kBlock, 0) != hipSuccess) return 1;
+    hipError_t e;
+    if (hub_dfs_in_lds(nlev, wmax))
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hub_dfs_kernel<true>, kBlock,
+                                                         hub_lds_bytes(nlev, wmax));
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hub_dfs_kernel<false>, kBlock, 0);
+    if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
 

@@ -24,12 +24,18 @@ struct KPlan {
     uint32_t smask[kMaxLevels];
 };
 
+// Per-vertex record read when the vertex appears in a root's row: where its
+// bitmap-build scan range starts in ocol and how long it is (one 16-byte
+// gather instead of three scattered ones).
+struct VInfo {
+    uint64_t hbeg;  // rowptr[v] (+ nlow[v] for ascending rows)
+    uint32_t hlen;  // neighbours with a larger key in the orientation's order
+    uint32_t deg;
+};
+
 struct KGraph {
-    const uint64_t *rowptr;  // n + 1
     const uint32_t *ocol;    // degree-oriented rows
-    const uint32_t *nlow;    // #neighbours with a smaller key
-    const uint32_t *hlen;    // bitmap-build scan length (starts at nlow if asc, else 0)
-    int32_t asc;
+    const VInfo *vinfo;      // n records
 };
 
 // Per-root record of an LDS batch, offsets relative to the batch.
@@ -52,7 +58,7 @@ struct HubRec {
     uint64_t joff;   // first global row index of this hub (bitmap build)
     uint64_t ioff;   // first global DFS item of this hub
     uint32_t jbeg;   // first x_1 of the hub's items
-    uint32_t pad;
+    uint32_t rlo;    // first bitmap row kept (rows below are never read)
 };
 
 // Donated level-2 subtree of a hub item: x_2 in C_2 n [lo, hi), with C_2

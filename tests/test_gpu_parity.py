@@ -193,3 +193,30 @@ def test_c1_sgm_hash(name):
     r, o, _ = run_pair(g, og, name, B.SGM)
     assert r["count"] == o["count"]
     assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
+
+
+def test_golden_brute_cases():
+    """Every tiny-graph case of tests/golden/brute.json (216 patterns x 8 graphs,
+    sgm and sgc) through the HIP path equals the brute-force expectation."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "brute.json")))
+    graphs = {}
+    for name, v in gold["graphs"].items():
+        e = sorted({(a, b) for a, b in v["edges"]} | {(b, a) for a, b in v["edges"]})
+        graphs[name] = B.Graph.from_edges([a for a, _ in e], [b for _, b in e])
+    plans = {}
+    bad = []
+    say("golden brute cases", len(gold["cases"]))
+    for c in gold["cases"]:
+        s, d = P.directed(gold["patterns"][c["pattern"]]["edges"])
+        for mode, key in ((B.SGM, "sgm"), (B.SGC, "sgc")):
+            pk = (c["pattern"], mode)
+            if pk not in plans:
+                plans[pk] = B.Plan.from_edges(s, d, mode)
+            pp = B.Prepared(graphs[c["graph"]], plans[pk])
+            pp.upload(0)
+            got = pp.count()["count"]
+            if got != c[key]:
+                bad.append((c["graph"], c["pattern"], key, got, c[key]))
+    assert not bad, bad[:10]
