@@ -16,7 +16,7 @@ from ._ffi import BeepError, check, lib
 SGM, SGC = 0, 1
 EMIT_NONE, EMIT_HASH, EMIT_BUFFER = 0, 1, 2
 
-__all__ = ["Graph", "Plan", "Prepared", "SG_Match", "SGM", "SGC", "BeepError",
+__all__ = ["Graph", "Plan", "Prepared", "Batch", "SG_Match", "SGM", "SGC", "BeepError",
            "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "tuple_hash", "mode_of"]
 
 
@@ -196,6 +196,47 @@ class Prepared:
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             lib().beep_prep_free(h)
+            self._h = C.c_void_p()
+
+
+def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0):
+    o = _ffi.Opts()
+    lib().beep_opts_default(C.byref(o))
+    o.device, o.emit, o.lanes = device, emit, lanes
+    o.donate = 1 if donate else 0
+    if donate_min:
+        o.donate_min = donate_min
+    return o
+
+
+class Batch:
+    """Several prepared queries run as one captured hipGraph (one persistent
+    kernel over all their LDS batches plus per-query hub branches)."""
+
+    def __init__(self, prepared, device=0, emit=EMIT_NONE, tune=None):
+        self.prepared = list(prepared)
+        n = len(self.prepared)
+        tune = tune or [{}] * n
+        self._opts = (_ffi.Opts * n)(*[_opts(device=device, emit=emit, lanes=t.get("lanes", 0),
+                                             donate_min=t.get("donate_min", 0)) for t in tune])
+        preps = (C.c_void_p * n)(*[p._h.value for p in self.prepared])
+        plans = (C.c_void_p * n)(*[p.plan._h.value for p in self.prepared])
+        h = C.c_void_p()
+        check(lib().beep_batch_create(preps, plans, self._opts, n, C.byref(h)))
+        self._h = h
+
+    def run(self):
+        n = len(self.prepared)
+        res = (_ffi.Result * n)()
+        t = C.c_double()
+        check(lib().beep_batch_run(self._h, res, C.byref(t)))
+        return [dict(count=r.count, hash_sum=r.hash_sum, hash_xor=r.hash_xor, donations=r.donations)
+                for r in res], t.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().beep_batch_free(h)
             self._h = C.c_void_p()
 
 

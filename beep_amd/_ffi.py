@@ -66,6 +66,10 @@ SIGNATURES = {
     "beep_prep_free": (None, [C.c_void_p]),
     "beep_opts_default": (None, [C.POINTER(Opts)]),
     "beep_count": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result)]),
+    "beep_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(Opts), C.c_int,
+                                    C.POINTER(C.c_void_p)]),
+    "beep_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Result), C.POINTER(C.c_double)]),
+    "beep_batch_free": (None, [C.c_void_p]),
     "beep_tuple_hash": (C.c_uint64, [u32p, C.c_int]),
     "beep_last_error": (C.c_char_p, []),
     "beep_version": (C.c_char_p, []),

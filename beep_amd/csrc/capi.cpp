@@ -18,6 +18,15 @@ struct beep_plan { std::unique_ptr<beep::Plan> p; };
 struct beep_prep { std::unique_ptr<beep::Prep> pp; };
 
 namespace beep {
+struct Batch;
+Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opts *opts, int n);
+void batch_run(Batch &b, beep_result *res, double *t_ms);
+void batch_free(Batch *b);
+}  // namespace beep
+
+struct beep_batch { beep::Batch *b; };
+
+namespace beep {
 
 void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res);
 void upload_prep(Prep &pp, int device, int skip_split);
@@ -255,6 +264,40 @@ int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_resul
         memset(out, 0, sizeof(*out));
         beep::count_on_device(*pp->pp, *p->p, opt, *out);
     });
+}
+
+int beep_batch_create(beep_prep *const *preps, const beep_plan *const *plans, const beep_opts *opts, int n,
+                      beep_batch **out)
+{
+    return guarded([&] {
+        if (!preps || !plans || !out || n <= 0) throw Error(BEEP_EINVAL, "null argument");
+        std::vector<beep::Prep *> pv(n);
+        std::vector<const beep::Plan *> lv(n);
+        std::vector<beep_opts> ov(n);
+        for (int i = 0; i < n; i++) {
+            if (!preps[i] || !plans[i]) throw Error(BEEP_EINVAL, "null query");
+            pv[i] = preps[i]->pp.get();
+            lv[i] = plans[i]->p.get();
+            if (opts) ov[i] = opts[i];
+            else beep_opts_default(&ov[i]);
+        }
+        *out = new beep_batch{beep::batch_create(pv.data(), lv.data(), ov.data(), n)};
+    });
+}
+
+int beep_batch_run(beep_batch *b, beep_result *results, double *t_ms)
+{
+    return guarded([&] {
+        if (!b) throw Error(BEEP_EINVAL, "null argument");
+        beep::batch_run(*b->b, results, t_ms);
+    });
+}
+
+void beep_batch_free(beep_batch *b)
+{
+    if (!b) return;
+    beep::batch_free(b->b);
+    delete b;
 }
 
 uint64_t beep_tuple_hash(const uint32_t *t, int len)

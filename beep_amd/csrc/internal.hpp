@@ -97,6 +97,8 @@ struct Plan {
     int above_split = 0;
     // Only bitmap rows >= split are ever read (rows of QEDGE positions).
     int rows_above_split = 0;
+    // Some level intersects with a bitmap row (QEDGE(l)[1:] non-empty).
+    int uses_bitmap = 0;
     uint16_t qmask[kMaxK] = {};  // positions QEDGE(l)[1:]
     uint16_t smask[kMaxK] = {};  // positions s in SYM(l), s > 0
 };
@@ -134,6 +136,7 @@ struct Prep {
     int reduced = 0;                  // rows start at the split (above_split plans)
     int skip_split = 0;               // plan's x_1 < split skip
     int rows_above_split = 0;         // hub bitmaps keep rows >= split only
+    int uses_bitmap = 1;              // hub bitmaps are built at all
     std::vector<uint32_t> wu, woff, wlen;
     uint32_t n_hub = 0;
     std::vector<LdsBatch> batches;

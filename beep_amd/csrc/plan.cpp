@@ -199,6 +199,9 @@ std::unique_ptr<Plan> plan_from_edges(const uint32_t *src, const uint32_t *dst, 
         for (int q = 1; q < l; q++)
             if (((p->qmask[l] >> q) & 1u) && !bound[q]) rows = false;
     p->rows_above_split = rows ? 1 : 0;
+    p->uses_bitmap = 0;
+    for (int l = 2; l < p->L; l++)
+        if (p->qmask[l]) p->uses_bitmap = 1;
     return p;
 }
 

@@ -195,6 +195,7 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
     pp->reduced = p.above_split;
     pp->skip_split = p.skip_split;
     pp->rows_above_split = p.rows_above_split;
+    pp->uses_bitmap = p.uses_bitmap;
     {
         std::vector<uint32_t> idx;
         idx.reserve(pp->roots.size());

@@ -25,6 +25,7 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
                               uint32_t b0, uint32_t b1, unsigned int *cursor, uint32_t ewords,
                               const KOut &out, int *grid_out);
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
@@ -222,8 +223,9 @@ static void upload(Prep &pp, int device, int p_skip_split)
         uint64_t ew = 0, hsl = 0;
         uint32_t h0 = 0;
         for (uint32_t h = 0; h < (uint32_t)hubs.size(); h++) {
-            uint64_t w = (uint64_t)(hubs[h].d - hubs[h].rlo) * ((hubs[h].d + 63) / 64);
-            uint64_t s = hubs[h].hslots;
+            // plans that never read a bitmap row need neither bitmap nor hash
+            uint64_t w = pp.uses_bitmap ? (uint64_t)(hubs[h].d - hubs[h].rlo) * ((hubs[h].d + 63) / 64) : 0;
+            uint64_t s = pp.uses_bitmap ? hubs[h].hslots : 0;
             if (h > h0 && (ew + w) * 8 + (hsl + s) * 8 > budget_bytes) {
                 d->rounds.push_back({h0, h, hubs[h0].joff, hubs[h].joff, hubs[h0].ioff, hubs[h].ioff, ew, hsl});
                 h0 = h;
@@ -269,26 +271,36 @@ static KPlan make_kplan(const Plan &p, int emit)
     return k;
 }
 
-void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res)
+// Everything one enumeration launch sequence needs; built (and buffers
+// allocated) before anything is enqueued, so the sequence can be captured.
+struct LaunchCfg {
+    KPlan kp;
+    KGraph kg;
+    KOut out;
+    HubQueue hq;
+    int hub_grid = 0;
+    int nlev = 1;
+    int lmin = 0;
+    bool donate = false;
+};
+
+static LaunchCfg prepare_launch(Prep &pp, const Plan &p, const beep_opts &o, uint32_t *dbuf)
 {
     upload(pp, o.device, p.skip_split);
     DevPrep &d = *pp.dev;
     HIP_CHECK(hipSetDevice(d.device));
-    if (p.min_qdeg != pp.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != pp.asc)
+    if (p.min_qdeg != pp.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != pp.asc ||
+        p.above_split != pp.reduced || p.rows_above_split != pp.rows_above_split)
         throw Error(BEEP_EINVAL, "plan does not match the prepared graph");
-    const int width = p.L;
-    uint32_t *dbuf = nullptr;
-    if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) dbuf = dev_alloc<uint32_t>(o.match_cap * (uint64_t)width);
-    KOut out{d.ctr + kCtrAcc, dbuf, (o.emit == BEEP_EMIT_BUFFER) ? o.match_cap : 0};
-    KPlan kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
-    KGraph kg{d.ocol, d.vinfo};
-    hipStream_t st = d.stream;
-    const int nlev = p.L > 3 ? p.L - 3 : 1;
-    // hub DFS stack slots
-    int hub_grid = hub_dfs_occupancy(nlev, std::max<uint32_t>(d.wmax, 1)) * d.cus;
+    LaunchCfg c;
+    c.out = KOut{d.ctr + kCtrAcc, dbuf, (o.emit == BEEP_EMIT_BUFFER) ? o.match_cap : 0};
+    c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
+    c.kg = KGraph{d.ocol, d.vinfo};
+    c.nlev = p.L > 3 ? p.L - 3 : 1;
+    c.hub_grid = hub_dfs_occupancy(c.nlev, std::max<uint32_t>(d.wmax, 1)) * d.cus;
     if (d.nhubs) {
         // per wave: nlev stack rows + the row of x_1 (used when they do not fit LDS)
-        size_t need = (size_t)hub_grid * (kBlock / 64) * (nlev + 1) * std::max<uint32_t>(d.wmax, 1);
+        size_t need = (size_t)c.hub_grid * (kBlock / 64) * (c.nlev + 1) * std::max<uint32_t>(d.wmax, 1);
         if (need > d.stk_words) {
             if (d.stk) HIP_CHECK(hipFree(d.stk));
             d.stk = dev_alloc<uint64_t>(need);
@@ -296,75 +308,111 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
         }
     }
     // donation queue (hub items with |C_2| > donate_min), allocated on first use
-    const bool donate = o.donate && d.nhubs && p.L >= 4;
-    if (donate && !d.queue) {
+    c.donate = o.donate && d.nhubs && p.L >= 4;
+    if (c.donate && !d.queue) {
         d.queue_cap = 1ull << 22;
         d.slot_cap = 1ull << 26;
         d.queue = dev_alloc<HubChunk>(d.queue_cap);
         d.slots = dev_alloc<uint64_t>(d.slot_cap);
     }
-    HubQueue hq{};
-    if (donate) {
-        hq.q = d.queue;
-        hq.cap = d.queue_cap;
-        hq.slots = d.slots;
-        hq.slot_cap = d.slot_cap;
-        hq.tail = d.ctr + kCtrQTail;
-        hq.head = d.ctr + kCtrQHead;
-        hq.slot_tail = d.ctr + kCtrSlot;
-        hq.donated = d.ctr + kCtrDonated;
-        hq.split_min = o.donate_min ? o.donate_min : 64;
-        hq.chunk = hq.split_min;
+    if (c.donate) {
+        c.hq.q = d.queue;
+        c.hq.cap = d.queue_cap;
+        c.hq.slots = d.slots;
+        c.hq.slot_cap = d.slot_cap;
+        c.hq.tail = d.ctr + kCtrQTail;
+        c.hq.head = d.ctr + kCtrQHead;
+        c.hq.slot_tail = d.ctr + kCtrSlot;
+        c.hq.donated = d.ctr + kCtrDonated;
+        c.hq.split_min = o.donate_min ? o.donate_min : 64;
+        c.hq.chunk = c.hq.split_min;
     }
-    HIP_CHECK(hipMemsetAsync(d.ctr, 0, sizeof(unsigned long long) * kCtrWords, st));
-    HIP_CHECK(hipEventRecord(d.ev[0], st));
-    // The hub rounds run on the main stream; the LDS lane classes are
-    // independent of them and of each other and run on the side streams.
-    for (auto &s : d.side) HIP_CHECK(hipStreamWaitEvent(s, d.ev[0], 0));
-    // LDS phase: consecutive batches with the same group width share a launch
-    int lmin = 0;
-    if (o.lanes > 0) {
-        while ((1 << lmin) < o.lanes && lmin < 6) lmin++;
+    c.lmin = 0;
+    if (o.lanes > 0)
+        while ((1 << c.lmin) < o.lanes && c.lmin < 6) c.lmin++;
+    return c;
+}
+
+// Hub rounds on one stream: clear, hash, bitmap rows, DFS + donation drain.
+static void enqueue_hubs(Prep &pp, const LaunchCfg &c, hipStream_t st)
+{
+    DevPrep &d = *pp.dev;
+    for (size_t ri = 0; ri < d.rounds.size(); ri++) {
+        const HubRound &r = d.rounds[ri];
+        const uint64_t rows = r.g1 - r.g0;
+        if (pp.uses_bitmap) {
+            HIP_CHECK(launch_fill_zero(d.arena, r.ewords * 8, st));
+            HIP_CHECK(launch_fill_zero(d.keys, r.hslots * 4, st));
+            int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)d.cus * 16);
+            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys, d.vals));
+            int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)d.cus * 16);
+            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys,
+                                        d.vals, d.arena));
+        }
+        unsigned long long *cur = d.ctr + kCtrHub + (ri % 64);
+        if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, 8, st));
+        const uint64_t items = r.i1 - r.i0;
+        if (items) {
+            if (c.donate && ri > 0) HIP_CHECK(launch_fill_zero(d.ctr + kCtrQTail, 3 * 8, st));
+            int g3 = (int)std::min<uint64_t>((items + 3) / 4, (uint64_t)c.hub_grid);
+            HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, c.kg, c.kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur,
+                                     d.arena, d.stk, std::max<uint32_t>(d.wmax, 1), c.out, c.hq, 0));
+            if (c.donate)
+                HIP_CHECK(launch_hub_dfs(c.hub_grid, st, c.kg, c.kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur,
+                                         d.arena, d.stk, std::max<uint32_t>(d.wmax, 1), c.out, c.hq, 1));
+        }
     }
+}
+
+// LDS lane classes, one launch each, round-robin over the given streams.
+static void enqueue_lds_classes(Prep &pp, const LaunchCfg &c, hipStream_t *streams, int nst)
+{
+    DevPrep &d = *pp.dev;
     uint32_t b = 0;
     int cls = 0;
     while (b < d.nbatches) {
-        int lg = std::max<int>(pp.batch_lanes_log2[b], lmin);
+        int lg = std::max<int>(pp.batch_lanes_log2[b], c.lmin);
         uint32_t e = b + 1;
-        while (e < d.nbatches && std::max<int>(pp.batch_lanes_log2[e], lmin) == lg) e++;
+        while (e < d.nbatches && std::max<int>(pp.batch_lanes_log2[e], c.lmin) == lg) e++;
         uint32_t ewords = std::min<uint32_t>(LdsBudget::kMaxEWords, LdsBudget::kMaxSumDeg << std::min(lg, 3));
         if (cls >= 32) throw Error(BEEP_ERANGE, "too many lane classes");
         unsigned int *cur = (unsigned int *)(d.ctr + kCtrLds) + cls;
-        HIP_CHECK(launch_lds_batches(lg, d.cus, d.side[cls % DevPrep::kSide], kg, kp, d.recs, d.batch_first,
-                                     d.batch_count, b, e, cur, ewords, out, nullptr));
+        HIP_CHECK(launch_lds_batches(lg, d.cus, streams[cls % nst], c.kg, c.kp, d.recs, d.batch_first,
+                                     d.batch_count, b, e, cur, ewords, c.out, nullptr));
         b = e;
         cls++;
     }
+}
+
+static void fill_result(const DevPrep &d, const beep_opts &o, const unsigned long long *acc, beep_result &res)
+{
+    res.count = acc[kCtrAcc + 0];
+    res.hash_sum = acc[kCtrAcc + 1];
+    res.hash_xor = acc[kCtrAcc + 2];
+    res.n_emitted = (o.emit != BEEP_EMIT_NONE) ? acc[0] : 0;
+    res.n_written = (o.emit == BEEP_EMIT_BUFFER && o.match_cap) ? std::min<uint64_t>(acc[3], o.match_cap) : 0;
+    res.donations = acc[kCtrDonated];
+    (void)d;
+}
+
+void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res)
+{
+    const int width = p.L;
+    uint32_t *dbuf = nullptr;
+    upload(pp, o.device, p.skip_split);
+    HIP_CHECK(hipSetDevice(pp.dev->device));
+    if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) dbuf = dev_alloc<uint32_t>(o.match_cap * (uint64_t)width);
+    LaunchCfg c = prepare_launch(pp, p, o, dbuf);
+    DevPrep &d = *pp.dev;
+    hipStream_t st = d.stream;
+    HIP_CHECK(launch_fill_zero(d.ctr, sizeof(unsigned long long) * kCtrWords, st));
+    HIP_CHECK(hipEventRecord(d.ev[0], st));
+    // The LDS lane classes run on the side streams, concurrently with the hub
+    // rounds on the main stream.
+    for (auto &s : d.side) HIP_CHECK(hipStreamWaitEvent(s, d.ev[0], 0));
+    enqueue_lds_classes(pp, c, d.side, DevPrep::kSide);
     for (int i = 0; i < DevPrep::kSide; i++) HIP_CHECK(hipEventRecord(d.side_done[i], d.side[i]));
-    // hub phase
-    for (size_t ri = 0; ri < d.rounds.size(); ri++) {
-        const HubRound &r = d.rounds[ri];
-        HIP_CHECK(hipMemsetAsync(d.arena, 0, r.ewords * 8, st));
-        HIP_CHECK(hipMemsetAsync(d.keys, 0, r.hslots * 4, st));
-        const uint64_t rows = r.g1 - r.g0;
-        int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)d.cus * 16);
-        HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys, d.vals));
-        int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)d.cus * 16);
-        HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys,
-                                    d.vals, d.arena));
-        unsigned long long *cur = d.ctr + kCtrHub + (ri % 64);
-        if (ri >= 64) HIP_CHECK(hipMemsetAsync(cur, 0, 8, st));
-        const uint64_t items = r.i1 - r.i0;
-        if (items) {
-            if (donate && ri > 0) HIP_CHECK(hipMemsetAsync(d.ctr + kCtrQTail, 0, 3 * 8, st));
-            int g3 = (int)std::min<uint64_t>((items + 3) / 4, (uint64_t)hub_grid);
-            HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, kg, kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur,
-                                     d.arena, d.stk, std::max<uint32_t>(d.wmax, 1), out, hq, 0));
-            if (donate)
-                HIP_CHECK(launch_hub_dfs(hub_grid, st, kg, kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur, d.arena,
-                                         d.stk, std::max<uint32_t>(d.wmax, 1), out, hq, 1));
-        }
-    }
+    enqueue_hubs(pp, c, st);
     HIP_CHECK(hipEventRecord(d.ev[1], st));
     for (int i = 0; i < DevPrep::kSide; i++) HIP_CHECK(hipStreamWaitEvent(st, d.side_done[i], 0));
     HIP_CHECK(hipEventRecord(d.ev[2], st));
@@ -380,15 +428,10 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
         HIP_CHECK(hipEventElapsedTime(&t, d.ev[0], d.side_done[i]));
         t_lds = std::max(t_lds, t);
     }
-    res.count = acc[0];
-    res.hash_sum = acc[1];
-    res.hash_xor = acc[2];
-    res.n_emitted = (o.emit != BEEP_EMIT_NONE) ? acc[0] : 0;
-    res.n_written = dbuf ? std::min<uint64_t>(acc[3], o.match_cap) : 0;
+    fill_result(d, o, acc, res);
     res.t_enum_ms = t_all;
     res.t_hub_ms = t_hub;
     res.t_lds_ms = t_lds;
-    res.donations = acc[kCtrDonated];
     if (dbuf) {
         if (o.match_buf && res.n_written)
             HIP_CHECK(hipMemcpy(o.match_buf, dbuf, res.n_written * width * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -397,5 +440,152 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
 }
 
 void upload_prep(Prep &pp, int device, int skip_split) { upload(pp, device, skip_split); }
+
+// ------------------------------------------------------------------ batch
+//
+// Several prepared queries in one hipGraph: per query a counter clear and its
+// hub rounds on a branch of their own, and one persistent lds_multi_kernel
+// over every query's LDS batches, heaviest first.
+
+struct Batch {
+    int device = -1;
+    std::vector<Prep *> preps;
+    std::vector<const Plan *> plans;
+    std::vector<beep_opts> opts;
+    std::vector<LaunchCfg> cfgs;
+    MultiQuery *dqs = nullptr;
+    MultiWork *dwork = nullptr;
+    uint32_t nwork = 0;
+    uint32_t nlev = 1;
+    unsigned int *cursor = nullptr;
+    hipStream_t s0 = nullptr;
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> done;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~Batch()
+    {
+        if (device >= 0) (void)hipSetDevice(device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : {(void *)dqs, (void *)dwork, (void *)cursor})
+            if (p) (void)hipFree(p);
+        for (auto e : done)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : {fork, t0, t1})
+            if (e) (void)hipEventDestroy(e);
+        for (auto s : branch)
+            if (s) (void)hipStreamDestroy(s);
+        if (s0) (void)hipStreamDestroy(s0);
+    }
+};
+
+hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work, uint32_t nwork,
+                            unsigned int *cursor, uint32_t nlev, int *grid_out);
+
+Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opts *opts, int n)
+{
+    if (n <= 0) throw Error(BEEP_EINVAL, "empty batch");
+    auto *b = new Batch();
+    try {
+        b->device = opts[0].device;
+        std::vector<MultiQuery> qs(n);
+        std::vector<std::pair<uint64_t, MultiWork>> work;
+        for (int i = 0; i < n; i++) {
+            if (opts[i].emit == BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "batches do not return tuple buffers");
+            if (opts[i].device != b->device) throw Error(BEEP_EINVAL, "a batch runs on one device");
+            b->preps.push_back(preps[i]);
+            b->plans.push_back(plans[i]);
+            b->opts.push_back(opts[i]);
+            b->cfgs.push_back(prepare_launch(*preps[i], *plans[i], opts[i], nullptr));
+            const LaunchCfg &c = b->cfgs.back();
+            Prep &pp = *preps[i];
+            DevPrep &d = *pp.dev;
+            qs[i] = MultiQuery{c.kg, c.kp, d.recs, d.batch_first, d.batch_count, c.out};
+            b->nlev = std::max<uint32_t>(b->nlev, (uint32_t)c.nlev);
+            for (uint32_t bi = 0; bi < (uint32_t)pp.batches.size(); bi++) {
+                const LdsBatch &lb = pp.batches[bi];
+                uint64_t cost = 0;
+                for (uint32_t r = 0; r < lb.count; r++) {
+                    uint64_t dd = pp.wlen[lb.first + r];
+                    cost += dd * dd;
+                }
+                uint32_t lg = std::max<uint32_t>(pp.batch_lanes_log2[bi], (uint32_t)c.lmin);
+                work.push_back({cost, MultiWork{(uint32_t)i, bi, lg, 0}});
+            }
+        }
+        std::stable_sort(work.begin(), work.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+        std::vector<MultiWork> w;
+        for (auto &x : work) w.push_back(x.second);
+        HIP_CHECK(hipSetDevice(b->device));
+        b->dqs = dev_upload(qs);
+        b->dwork = dev_upload(w);
+        b->nwork = (uint32_t)w.size();
+        b->cursor = dev_alloc<unsigned int>(1);
+        HIP_CHECK(hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking));
+        b->branch.resize(n + 1);
+        b->done.resize(n + 1);
+        for (auto &s : b->branch) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        for (auto &e : b->done) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreate(&b->t0));
+        HIP_CHECK(hipEventCreate(&b->t1));
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+static void batch_capture(Batch &b)
+{
+    const int n = (int)b.preps.size();
+    HIP_CHECK(hipStreamBeginCapture(b.s0, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n; i++)
+        HIP_CHECK(launch_fill_zero(b.preps[i]->dev->ctr, sizeof(unsigned long long) * kCtrWords, b.s0));
+    HIP_CHECK(launch_fill_zero(b.cursor, 4, b.s0));
+    HIP_CHECK(hipEventRecord(b.fork, b.s0));
+    for (int i = 0; i < n; i++) {
+        HIP_CHECK(hipStreamWaitEvent(b.branch[i], b.fork, 0));
+        enqueue_hubs(*b.preps[i], b.cfgs[i], b.branch[i]);
+        HIP_CHECK(hipEventRecord(b.done[i], b.branch[i]));
+    }
+    HIP_CHECK(hipStreamWaitEvent(b.branch[n], b.fork, 0));
+    const DevPrep &d0 = *b.preps[0]->dev;
+    if (b.nwork)
+        HIP_CHECK(launch_lds_multi(d0.cus, b.branch[n], b.dqs, b.dwork, b.nwork, b.cursor, b.nlev, nullptr));
+    HIP_CHECK(hipEventRecord(b.done[n], b.branch[n]));
+    for (int i = 0; i <= n; i++) HIP_CHECK(hipStreamWaitEvent(b.s0, b.done[i], 0));
+    HIP_CHECK(hipStreamEndCapture(b.s0, &b.graph));
+    HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+}
+
+void batch_run(Batch &b, beep_result *res, double *t_ms)
+{
+    HIP_CHECK(hipSetDevice(b.device));
+    if (!b.exec) batch_capture(b);
+    HIP_CHECK(hipEventRecord(b.t0, b.s0));
+    HIP_CHECK(hipGraphLaunch(b.exec, b.s0));
+    HIP_CHECK(hipEventRecord(b.t1, b.s0));
+    const int n = (int)b.preps.size();
+    std::vector<unsigned long long> acc((size_t)n * kCtrWords);
+    for (int i = 0; i < n; i++)
+        HIP_CHECK(hipMemcpyAsync(acc.data() + (size_t)i * kCtrWords, b.preps[i]->dev->ctr,
+                                 sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, b.s0));
+    HIP_CHECK(hipStreamSynchronize(b.s0));
+    float t = 0;
+    HIP_CHECK(hipEventElapsedTime(&t, b.t0, b.t1));
+    for (int i = 0; i < n; i++) {
+        if (res) {
+            memset(&res[i], 0, sizeof(beep_result));
+            fill_result(*b.preps[i]->dev, b.opts[i], acc.data() + (size_t)i * kCtrWords, res[i]);
+            res[i].t_enum_ms = t;
+        }
+    }
+    if (t_ms) *t_ms = t;
+}
+
+void batch_free(Batch *b) { delete b; }
 
 }  // namespace beep

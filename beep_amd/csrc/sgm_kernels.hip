@@ -209,88 +209,101 @@ __device__ void block_excl_scan(uint32_t *a, uint32_t n, uint32_t *tmp4)
     __syncthreads();
 }
 
-template <int P>
-__global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
-                                                           const RootRec *__restrict__ recs,
-                                                           const uint32_t *__restrict__ batch_first,
-                                                           const uint32_t *__restrict__ batch_count,
-                                                           uint32_t b0, uint32_t b1,
-                                                           unsigned int *cursor, LdsLayout lay,
-                                                           KOut out)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    uint64_t *sE = (uint64_t *)(smem + lay.oE);
-    uint64_t *sHBeg = (uint64_t *)(smem + lay.oHBeg);
-    uint64_t *sStk = (uint64_t *)(smem + lay.oStk);
-    RootRec *sRoot = (RootRec *)(smem + lay.oRoot);
-    uint32_t *sA = (uint32_t *)(smem + lay.oA);
-    uint32_t *sPre = (uint32_t *)(smem + lay.oPre);
-    uint32_t *sHKey = (uint32_t *)(smem + lay.oHKey);
-    uint32_t *sMisc = (uint32_t *)(smem + lay.oMisc);
-    uint16_t *sHVal = (uint16_t *)(smem + lay.oHVal);
-    uint16_t *sX = (uint16_t *)(smem + lay.oX);
-    uint8_t *sJRoot = (uint8_t *)(smem + lay.oJRoot);
+// LDS carve-up of one workgroup.
+struct LdsView {
+    uint64_t *sE, *sHBeg, *sStk;
+    RootRec *sRoot;
+    uint32_t *sA, *sPre, *sHKey, *sMisc;
+    uint16_t *sHVal, *sX;
+    uint8_t *sJRoot;
+};
 
+__device__ __forceinline__ LdsView lds_view(unsigned char *smem, const LdsLayout &lay)
+{
+    LdsView v;
+    v.sE = (uint64_t *)(smem + lay.oE);
+    v.sHBeg = (uint64_t *)(smem + lay.oHBeg);
+    v.sStk = (uint64_t *)(smem + lay.oStk);
+    v.sRoot = (RootRec *)(smem + lay.oRoot);
+    v.sA = (uint32_t *)(smem + lay.oA);
+    v.sPre = (uint32_t *)(smem + lay.oPre);
+    v.sHKey = (uint32_t *)(smem + lay.oHKey);
+    v.sMisc = (uint32_t *)(smem + lay.oMisc);
+    v.sHVal = (uint16_t *)(smem + lay.oHVal);
+    v.sX = (uint16_t *)(smem + lay.oX);
+    v.sJRoot = (uint8_t *)(smem + lay.oJRoot);
+    return v;
+}
+
+// Processes one LDS batch: stage rows, hash and bitmaps, then the DFS over
+// its (root, x_1) items.  PC > 0 fixes the group width at compile time; PC == 0
+// takes it from Prt.  The plan's level masks are in sMisc[M_QM..], [M_SM..].
+template <int PC>
+__device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v,
+                              const RootRec *__restrict__ recs, uint32_t first, uint32_t nr,
+                              const KOut &out, int Prt, uint64_t &cnt, uint64_t &hs, uint64_t &hx)
+{
+    const int P = PC > 0 ? PC : Prt;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int lx = lane & (P - 1);
     const int pbase = lane & ~(P - 1);
     const int L = pl.L;
-
-    if (tid < kMaxLevels) {
-        sMisc[M_QM + tid] = pl.qmask[tid];
-        sMisc[M_SM + tid] = pl.smask[tid];
-    }
-
-    uint64_t cnt = 0, hs = 0, hx = 0;
+    uint64_t *sE = v.sE;
+    uint64_t *sHBeg = v.sHBeg;
+    uint64_t *sStk = v.sStk;
+    RootRec *sRoot = v.sRoot;
+    uint32_t *sA = v.sA, *sPre = v.sPre, *sHKey = v.sHKey, *sMisc = v.sMisc;
+    uint16_t *sHVal = v.sHVal, *sX = v.sX;
+    uint8_t *sJRoot = v.sJRoot;
     auto xget = [&](int q) -> uint32_t { return sX[q * kBlock + tid]; };
+    // bitmaps are built only when some level intersects with a bitmap row
+    bool needE = false;
+    for (int l = 2; l < L; l++) needE |= sMisc[M_QM + l] != 0u;
 
-    for (;;) {
-        if (tid == 0) sMisc[M_BATCH] = b0 + atomicAdd(cursor, 1u);
-        __syncthreads();
-        const uint32_t b = sMisc[M_BATCH];
-        if (b >= b1) break;
-        const uint32_t first = batch_first[b], nr = batch_count[b];
-        if (tid < (int)nr) sRoot[tid] = recs[first + tid];
-        __syncthreads();
-        const RootRec last = sRoot[nr - 1];
-        const uint32_t sumd = last.doff + last.d;
-        const uint32_t ew = last.eoff + last.d * ((last.d + 63) >> 6);
-        const uint32_t hsl = last.hoff + hash_slots_dev(last.d);
+    if (tid < (int)nr) sRoot[tid] = recs[first + tid];
+    __syncthreads();
+    const RootRec last = sRoot[nr - 1];
+    const uint32_t sumd = last.doff + last.d;
+    const uint32_t ew = last.eoff + last.d * ((last.d + 63) >> 6);
+    const uint32_t hsl = last.hoff + hash_slots_dev(last.d);
+    if (needE) {
         for (uint32_t i = tid; i < ew; i += kBlock) sE[i] = 0;
         for (uint32_t i = tid; i < hsl; i += kBlock) sHKey[i] = 0;
-        if (tid == 0) sMisc[M_NEXT] = 0;
-        __syncthreads();
+    }
+    if (tid == 0) sMisc[M_NEXT] = 0;
+    __syncthreads();
 
-        // rows, hash tables and "higher" ranges of every row vertex
-        for (uint32_t j = tid; j < sumd; j += kBlock) {
-            uint32_t lo = 0, hi = nr;
-            while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (sRoot[mid].doff <= j) lo = mid; else hi = mid;
-            }
-            const RootRec rr = sRoot[lo];
-            const uint32_t jl = j - rr.doff;
-            const uint32_t v = g.ocol[rr.rbeg + jl];
-            sA[j] = v;
-            sJRoot[j] = (uint8_t)lo;
-            const uint32_t H = hash_slots_dev(rr.d);
-            uint32_t s = hash_slot(v, H);
-            for (;;) {
-                if (atomicCAS(&sHKey[rr.hoff + s], 0u, v + 1u) == 0u) {
-                    sHVal[rr.hoff + s] = (uint16_t)jl;
-                    break;
-                }
-                s = (s + 1) & (H - 1);
-            }
-            const VInfo vi = g.vinfo[v];
-            sHBeg[j] = vi.hbeg;
-            sPre[j] = vi.hlen;
+    // rows, hash tables and "higher" ranges of every row vertex
+    for (uint32_t j = tid; j < sumd; j += kBlock) {
+        uint32_t lo = 0, hi = nr;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (sRoot[mid].doff <= j) lo = mid; else hi = mid;
         }
-        __syncthreads();
+        const RootRec rr = sRoot[lo];
+        const uint32_t jl = j - rr.doff;
+        const uint32_t vx = g.ocol[rr.rbeg + jl];
+        sA[j] = vx;
+        sJRoot[j] = (uint8_t)lo;
+        if (!needE) continue;
+        const uint32_t H = hash_slots_dev(rr.d);
+        uint32_t s = hash_slot(vx, H);
+        for (;;) {
+            if (atomicCAS(&sHKey[rr.hoff + s], 0u, vx + 1u) == 0u) {
+                sHVal[rr.hoff + s] = (uint16_t)jl;
+                break;
+            }
+            s = (s + 1) & (H - 1);
+        }
+        const VInfo vi = g.vinfo[vx];
+        sHBeg[j] = vi.hbeg;
+        sPre[j] = vi.hlen;
+    }
+    __syncthreads();
+    if (needE) {
         block_excl_scan(sPre, sumd, sMisc + 28);
         const uint32_t total = sPre[sumd];
-
         // bitmap: each adjacent pair (v, w), key(v) < key(w), is found from v
         for (uint32_t e = tid; e < total; e += kBlock) {
             uint32_t lo = 0, hi = sumd;  // largest j with sPre[j] <= e
@@ -317,83 +330,141 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
             }
         }
         __syncthreads();
+    }
 
-        // DFS over (root, x_1) items
+    // DFS over (root, x_1) items
+    for (;;) {
+        uint32_t item = 0;
+        if (lx == 0) item = atomicAdd(&sMisc[M_NEXT], 1u);
+        item = __shfl(item, 0, P);
+        if (item >= sumd) break;
+        const RootRec rr = sRoot[sJRoot[item]];
+        const uint32_t jl = item - rr.doff;
+        if (pl.skip_split && jl < rr.split) continue;
+        if (L < 3) continue;
+        const uint32_t d = rr.d, W = (d + 63) >> 6;
+        const uint64_t *E = sE + rr.eoff;
+        const uint32_t *A = sA + rr.doff;
+        const uint32_t w = (uint32_t)lx;
+        const bool own = w < W;
+        sX[1 * kBlock + tid] = (uint16_t)jl;
+
+        auto level_word = [&](int l) -> uint64_t {
+            uint64_t t = valid_mask(w, d);
+            uint32_t qm = sMisc[M_QM + l];
+            while (qm) {
+                const int q = __builtin_ctz(qm);
+                qm &= qm - 1;
+                t &= E[xget(q) * W + w];
+            }
+            for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
+            uint32_t sm = sMisc[M_SM + l];
+            while (sm) {
+                const int s = __builtin_ctz(sm);
+                sm &= sm - 1;
+                t &= gt_mask(xget(s), w);
+            }
+            return t;
+        };
+
+        const uint64_t t2 = own ? level_word(2) : 0ull;
+        if (L == 3) {
+            cnt += (uint64_t)__popcll(t2);
+            if (pl.emit && t2) emit_word(out, pl, 2, rr.u, A, t2, w, xget, hs, hx);
+            continue;
+        }
+        sStk[tid] = t2;
+        int l = 3;
+        uint32_t pos = 0;
         for (;;) {
-            uint32_t item = 0;
-            if (lx == 0) item = atomicAdd(&sMisc[M_NEXT], 1u);
-            item = __shfl(item, 0, P);
-            if (item >= sumd) break;
-            const RootRec rr = sRoot[sJRoot[item]];
-            const uint32_t jl = item - rr.doff;
-            if (pl.skip_split && jl < rr.split) continue;
-            if (L < 3) continue;
-            const uint32_t d = rr.d, W = (d + 63) >> 6;
-            const uint64_t *E = sE + rr.eoff;
-            const uint32_t *A = sA + rr.doff;
-            const uint32_t w = (uint32_t)lx;
-            const bool own = w < W;
-            sX[1 * kBlock + tid] = (uint16_t)jl;
-
-            auto level_word = [&](int l) -> uint64_t {
-                uint64_t t = valid_mask(w, d);
-                uint32_t qm = sMisc[M_QM + l];
-                while (qm) {
-                    const int q = __builtin_ctz(qm);
-                    qm &= qm - 1;
-                    t &= E[xget(q) * W + w];
-                }
-                for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
-                uint32_t sm = sMisc[M_SM + l];
-                while (sm) {
-                    const int s = __builtin_ctz(sm);
-                    sm &= sm - 1;
-                    t &= gt_mask(xget(s), w);
-                }
-                return t;
-            };
-
-            const uint64_t t2 = own ? level_word(2) : 0ull;
-            if (L == 3) {
-                cnt += (uint64_t)__popcll(t2);
-                if (pl.emit && t2) emit_word(out, pl, 2, rr.u, A, t2, w, xget, hs, hx);
+            uint64_t vv = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
+            const uint32_t pw = pos >> 6;
+            if (w < pw) vv = 0ull;
+            else if (w == pw) vv &= (~0ull << (pos & 63u));
+            const uint64_t bal = __ballot(vv != 0ull);
+            const uint64_t mine = (P == 64) ? bal : ((bal >> pbase) & ((1ull << P) - 1ull));
+            if (mine == 0ull) {
+                if (l == 3) break;
+                --l;
+                pos = (uint32_t)xget(l - 1) + 1u;
                 continue;
             }
-            sStk[tid] = t2;
-            int l = 3;
-            uint32_t pos = 0;
-            for (;;) {
-                uint64_t v = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
-                const uint32_t pw = pos >> 6;
-                if (w < pw) v = 0ull;
-                else if (w == pw) v &= (~0ull << (pos & 63u));
-                const uint64_t bal = __ballot(v != 0ull);
-                const uint64_t mine = (P == 64) ? bal : ((bal >> pbase) & ((1ull << P) - 1ull));
-                if (mine == 0ull) {
-                    if (l == 3) break;
-                    --l;
-                    pos = (uint32_t)xget(l - 1) + 1u;
-                    continue;
-                }
-                const int f = __builtin_ctzll(mine);
-                const uint64_t vf = __shfl(v, f, P);
-                const uint32_t xi = ((uint32_t)f << 6) + (uint32_t)__builtin_ctzll(vf);
-                sX[(l - 1) * kBlock + tid] = (uint16_t)xi;
-                const uint64_t t = own ? level_word(l) : 0ull;
-                if (l == L - 1) {
-                    cnt += (uint64_t)__popcll(t);
-                    if (pl.emit && t) emit_word(out, pl, l, rr.u, A, t, w, xget, hs, hx);
-                    pos = xi + 1u;
-                } else {
-                    sStk[(l - 2) * kBlock + tid] = t;
-                    ++l;
-                    pos = 0;
-                }
+            const int f = __builtin_ctzll(mine);
+            const uint64_t vf = __shfl(vv, f, P);
+            const uint32_t xi = ((uint32_t)f << 6) + (uint32_t)__builtin_ctzll(vf);
+            sX[(l - 1) * kBlock + tid] = (uint16_t)xi;
+            const uint64_t t = own ? level_word(l) : 0ull;
+            if (l == L - 1) {
+                cnt += (uint64_t)__popcll(t);
+                if (pl.emit && t) emit_word(out, pl, l, rr.u, A, t, w, xget, hs, hx);
+                pos = xi + 1u;
+            } else {
+                sStk[(l - 2) * kBlock + tid] = t;
+                ++l;
+                pos = 0;
             }
         }
+    }
+    __syncthreads();
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
+                                                           const RootRec *__restrict__ recs,
+                                                           const uint32_t *__restrict__ batch_first,
+                                                           const uint32_t *__restrict__ batch_count,
+                                                           uint32_t b0, uint32_t b1,
+                                                           unsigned int *cursor, LdsLayout lay,
+                                                           KOut out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const LdsView v = lds_view(smem, lay);
+    const int tid = threadIdx.x;
+    if (tid < kMaxLevels) {
+        v.sMisc[M_QM + tid] = pl.qmask[tid];
+        v.sMisc[M_SM + tid] = pl.smask[tid];
+    }
+    uint64_t cnt = 0, hs = 0, hx = 0;
+    for (;;) {
+        if (tid == 0) v.sMisc[M_BATCH] = b0 + atomicAdd(cursor, 1u);
         __syncthreads();
+        const uint32_t b = v.sMisc[M_BATCH];
+        if (b >= b1) break;
+        process_batch<P>(g, pl, v, recs, batch_first[b], batch_count[b], out, P, cnt, hs, hx);
     }
     flush_acc(out, cnt, hs, hx);
+}
+
+// One persistent launch over the LDS batches of several prepared queries.
+// Work records are ordered heaviest first; a workgroup takes the next record,
+// loads that query's plan and graph and runs the batch with the record's
+// group width, then flushes its counts to that query's counters.
+__global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__restrict__ qs,
+                                                           const MultiWork *__restrict__ work,
+                                                           uint32_t nwork, unsigned int *cursor,
+                                                           LdsLayout lay)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const LdsView v = lds_view(smem, lay);
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) v.sMisc[M_BATCH] = atomicAdd(cursor, 1u);
+        __syncthreads();
+        const uint32_t wi = v.sMisc[M_BATCH];
+        if (wi >= nwork) break;
+        const MultiWork w = work[wi];
+        const MultiQuery &q = qs[w.query];
+        const KPlan pl = q.pl;
+        if (tid < kMaxLevels) {
+            v.sMisc[M_QM + tid] = pl.qmask[tid];
+            v.sMisc[M_SM + tid] = pl.smask[tid];
+        }
+        __syncthreads();
+        uint64_t cnt = 0, hs = 0, hx = 0;
+        process_batch<0>(q.g, pl, v, q.recs, q.batch_first[w.batch], q.batch_count[w.batch], q.out,
+                         1 << w.lanes_log2, cnt, hs, hx);
+        flush_acc(q.out, cnt, hs, hx);
+    }
 }
 
 // ------------------------------------------------------------------- hubs
@@ -815,6 +886,40 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
 #undef BEEP_CASE
     }
     return hipErrorInvalidValue;
+}
+
+// Zero fill (used instead of memset nodes so launch sequences can be captured).
+__global__ void fill_zero_u32_kernel(uint32_t *p, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st)
+{
+    const uint64_t n = (bytes + 3) / 4;
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_zero_u32_kernel, dim3((uint32_t)blocks), dim3(kBlock), 0, st, (uint32_t *)p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work,
+                            uint32_t nwork, unsigned int *cursor, uint32_t nlev, int *grid_out)
+{
+    LdsLayout lay = make_lds_layout(LdsBudget::kMaxEWords, nlev);
+    size_t lds = lay.bytes;
+    (void)hipFuncSetAttribute((const void *)lds_multi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, lds_multi_kernel, kBlock, lds) != hipSuccess || occ < 1)
+        occ = 1;
+    int grid = occ * cus;
+    if ((uint32_t)grid > nwork) grid = (int)nwork;
+    if (grid < 1) grid = 1;
+    if (grid_out) *grid_out = grid;
+    hipLaunchKernelGGL(lds_multi_kernel, dim3(grid), dim3(kBlock), lds, st, qs, work, nwork, cursor, lay);
+    return hipGetLastError();
 }
 
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,

@@ -89,4 +89,22 @@ struct KOut {
     uint64_t cap;
 };
 
+// One prepared query of a multi-query launch.
+struct MultiQuery {
+    KGraph g;
+    KPlan pl;
+    const RootRec *recs;
+    const uint32_t *batch_first;
+    const uint32_t *batch_count;
+    KOut out;
+};
+
+// One LDS batch of one query.
+struct MultiWork {
+    uint32_t query;
+    uint32_t batch;
+    uint32_t lanes_log2;
+    uint32_t pad;
+};
+
 }  // namespace beep

@@ -20,7 +20,7 @@ WORKLOADS = {
                    queries=[("triangle", "sgc")]),
     "chunglu1m": dict(kind="chung_lu", n=1134890, samples=3070000, gamma=2.2, i0=3.5, seed=2,
                       queries=[("4clique", "sgm"), ("diamond", "sgm")]),
-    "rmat22cut": dict(kind="rmat", scale=22, samples=19_500_000, seed=3, n_keep=3774768,
+    "rmat22cut": dict(kind="rmat", scale=22, samples=17_030_000, seed=3, n_keep=3774768,
                       queries=[("triangle", "sgc"), ("tailed_triangle", "sgc"), ("diamond", "sgc"),
                                ("4clique", "sgc"), ("4star", "sgc"), ("w5", "sgc"),
                                ("5clique", "sgc"), ("6clique", "sgc")]),

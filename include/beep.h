@@ -175,6 +175,24 @@ typedef struct {
 void beep_opts_default(beep_opts *o);
 int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_result *out);
 
+/* ------------------------------------------------------------------------ */
+/* Batches.  No counterpart in the reference (one query per process there):  */
+/* several prepared queries over resident graphs run as one captured         */
+/* hipGraph -- per query its hub rounds on a branch of their own, and one    */
+/* persistent kernel over every query's LDS batches.  The graph is captured  */
+/* on the first run and replayed afterwards.                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct beep_batch beep_batch;
+
+/* opts: n entries or NULL (defaults); every query must use the same device;
+ * BEEP_EMIT_BUFFER is not supported in a batch. */
+int beep_batch_create(beep_prep *const *preps, const beep_plan *const *plans, const beep_opts *opts,
+                      int n, beep_batch **out);
+/* results: n entries (t_enum_ms is the time of the whole graph launch). */
+int beep_batch_run(beep_batch *b, beep_result *results, double *t_ms);
+void beep_batch_free(beep_batch *b);
+
 /* Tuple hash used by BEEP_EMIT_HASH: sum and xor over emitted tuples. */
 uint64_t beep_tuple_hash(const uint32_t *t, int len);
 

@@ -220,3 +220,28 @@ def test_golden_brute_cases():
             if got != c[key]:
                 bad.append((c["graph"], c["pattern"], key, got, c[key]))
     assert not bad, bad[:10]
+
+
+@pytest.mark.parametrize("mode", [B.SGM, B.SGC])
+def test_batch_graph_matches_single_queries(cl_graph, mode):
+    """One captured graph over 7 queries gives each query's single-launch result,
+    on every replay, with eager launches interleaved between replays."""
+    g, og = cl_graph
+    names = ["triangle", "tailed_triangle", "diamond", "4clique", "4star", "w5", "5clique"]
+    say("batch", mode)
+    preps = []
+    for name in names:
+        s, d = P.directed(P.read_mtx_edges(name))
+        pp = B.Prepared(g, B.Plan.from_edges(s, d, mode))
+        pp.upload(0)
+        preps.append(pp)
+    single = [pp.count(emit=B.EMIT_HASH) for pp in preps]
+    batch = B.Batch(preps, emit=B.EMIT_HASH)
+    for rep in range(3):
+        res, t = batch.run()
+        for name, r, s1 in zip(names, res, single):
+            assert (r["count"], r["hash_sum"], r["hash_xor"]) == (s1["count"], s1["hash_sum"], s1["hash_xor"]), (rep, name)
+        preps[rep].count()  # eager work between replays
+    s, d = P.directed(P.read_mtx_edges("diamond"))
+    o = O.Prepared(og, O.plan(s, d, mode)).count(want_hash=True)
+    assert single[2]["count"] == o["count"]
