@@ -16,8 +16,10 @@ from ._ffi import BeepError, check, lib
 SGM, SGC = 0, 1
 EMIT_NONE, EMIT_HASH, EMIT_BUFFER = 0, 1, 2
 
-__all__ = ["Graph", "Plan", "Prepared", "Batch", "SG_Match", "SGM", "SGC", "BeepError",
-           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "tuple_hash", "mode_of"]
+__all__ = ["Graph", "Plan", "Prepared", "Batch", "Server", "SG_Match", "SGM", "SGC", "BeepError",
+           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "ETIMEDOUT", "tuple_hash", "mode_of"]
+
+ETIMEDOUT = -110
 
 
 def mode_of(task):
@@ -171,23 +173,21 @@ class Prepared:
     def upload(self, device=0):
         check(lib().beep_prep_upload(self._h, device))
 
-    def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0):
-        o = _ffi.Opts()
-        lib().beep_opts_default(C.byref(o))
-        o.device, o.emit, o.lanes = device, emit, lanes
-        o.donate = 1 if donate else 0
-        if donate_min:
-            o.donate_min = donate_min
+    def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0,
+              deadline_ms=0.0):
+        """Runs the enumeration.  With a deadline the result may be partial:
+        then ``stopped`` is 1 (the C call's BEEP_ETIMEDOUT is not raised)."""
+        o = _opts(device, emit, lanes, donate, donate_min, deadline_ms)
         buf = None
         if emit == EMIT_BUFFER and match_cap:
             buf = np.zeros(match_cap * self.plan.width, dtype=np.uint32)
             o.match_cap = match_cap
             o.match_buf = buf.ctypes.data_as(_ffi.u32p)
         r = _ffi.Result()
-        check(lib().beep_count(self._h, self.plan._h, C.byref(o), C.byref(r)))
-        out = dict(count=r.count, hash_sum=r.hash_sum, hash_xor=r.hash_xor,
-                   emitted=r.n_emitted, written=r.n_written, t_enum_ms=r.t_enum_ms,
-                   t_hub_ms=r.t_hub_ms, t_lds_ms=r.t_lds_ms, donations=r.donations)
+        rc = lib().beep_count(self._h, self.plan._h, C.byref(o), C.byref(r))
+        if rc != ETIMEDOUT:
+            check(rc)
+        out = _result_dict(r)
         if buf is not None:
             out["matches"] = buf[:r.n_written * self.plan.width].reshape(-1, self.plan.width)
         return out
@@ -199,14 +199,22 @@ class Prepared:
             self._h = C.c_void_p()
 
 
-def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0):
+def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline_ms=0.0):
     o = _ffi.Opts()
     lib().beep_opts_default(C.byref(o))
     o.device, o.emit, o.lanes = device, emit, lanes
     o.donate = 1 if donate else 0
     if donate_min:
         o.donate_min = donate_min
+    o.deadline_ms = float(deadline_ms)
     return o
+
+
+def _result_dict(r):
+    return dict(count=r.count, hash_sum=r.hash_sum, hash_xor=r.hash_xor,
+                emitted=r.n_emitted, written=r.n_written, t_enum_ms=r.t_enum_ms,
+                t_hub_ms=r.t_hub_ms, t_lds_ms=r.t_lds_ms, donations=r.donations,
+                stopped=r.stopped)
 
 
 class Batch:
@@ -230,14 +238,80 @@ class Batch:
         res = (_ffi.Result * n)()
         t = C.c_double()
         check(lib().beep_batch_run(self._h, res, C.byref(t)))
-        return [dict(count=r.count, hash_sum=r.hash_sum, hash_xor=r.hash_xor, donations=r.donations)
-                for r in res], t.value
+        return [_result_dict(r) for r in res], t.value
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             lib().beep_batch_free(h)
             self._h = C.c_void_p()
+
+
+class Server:
+    """Serving loop over one resident data graph (libbeep's beep_server).
+
+    ``submit`` returns a ticket at once; a worker thread in the library admits
+    queries in arrival order as lanes and scratch memory free up and runs them
+    concurrently; ``poll`` returns finished queries as dicts with the counts,
+    ``status`` (0, ETIMEDOUT when the deadline stopped it) and the queue,
+    service and end-to-end latencies in ms.
+    """
+
+    def __init__(self, graph, device=0, lanes=4, pool_bytes=0, arena_cap=0):
+        o = _ffi.ServerOpts()
+        lib().beep_server_opts_default(C.byref(o))
+        o.device, o.lanes, o.pool_bytes, o.arena_cap = device, lanes, pool_bytes, arena_cap
+        h = C.c_void_p()
+        check(lib().beep_server_create(graph._h, C.byref(o), C.byref(h)))
+        self._h = h
+        self.graph = graph      # the library keeps a pointer to it
+        self._keep = {}         # ticket -> (plan, opts, buffer) alive until polled
+
+    def prepare(self, plan):
+        check(lib().beep_server_prepare(self._h, plan._h))
+
+    def submit(self, plan, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0,
+               deadline_ms=0.0):
+        o = _opts(0, emit, lanes, donate, donate_min, deadline_ms)
+        buf = None
+        if emit == EMIT_BUFFER and match_cap:
+            buf = np.zeros(match_cap * plan.width, dtype=np.uint32)
+            o.match_cap = match_cap
+            o.match_buf = buf.ctypes.data_as(_ffi.u32p)
+        t = C.c_uint64()
+        check(lib().beep_server_submit(self._h, plan._h, C.byref(o), C.byref(t)))
+        self._keep[t.value] = (plan, o, buf)
+        return t.value
+
+    def poll(self, timeout_ms=-1):
+        """One finished query, or None when none finished within timeout_ms."""
+        d = _ffi.ServerDone()
+        rc = lib().beep_server_poll(self._h, C.byref(d), int(timeout_ms))
+        if rc == ETIMEDOUT:
+            return None
+        check(rc)
+        plan, _, buf = self._keep.pop(d.ticket, (None, None, None))
+        out = _result_dict(d.result)
+        out.update(ticket=d.ticket, status=d.status, queue_ms=d.queue_ms, service_ms=d.service_ms,
+                   latency_ms=d.latency_ms, footprint=d.footprint)
+        if buf is not None and plan is not None:
+            out["matches"] = buf[:d.result.n_written * plan.width].reshape(-1, plan.width)
+        return out
+
+    def stats(self):
+        s = _ffi.ServerStats()
+        check(lib().beep_server_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _ffi.ServerStats._fields_}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().beep_server_free(h)
+            self._h = C.c_void_p()
+        self._keep.clear()
+
+    def __del__(self):
+        self.close()
 
 
 class SG_Match:

@@ -36,13 +36,31 @@ class PrepDesc(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("emit", C.c_int), ("match_cap", C.c_uint64),
                 ("match_buf", u32p), ("lanes", C.c_int), ("donate", C.c_int),
-                ("donate_min", C.c_uint32), ("timing", C.c_int)]
+                ("donate_min", C.c_uint32), ("timing", C.c_int), ("deadline_ms", C.c_double)]
 
 
 class Result(C.Structure):
     _fields_ = [("count", C.c_uint64), ("hash_sum", C.c_uint64), ("hash_xor", C.c_uint64),
                 ("n_emitted", C.c_uint64), ("n_written", C.c_uint64), ("t_enum_ms", C.c_double),
-                ("t_hub_ms", C.c_double), ("t_lds_ms", C.c_double), ("donations", C.c_uint64)]
+                ("t_hub_ms", C.c_double), ("t_lds_ms", C.c_double), ("donations", C.c_uint64),
+                ("stopped", C.c_uint64)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("lanes", C.c_int), ("pool_bytes", C.c_uint64),
+                ("arena_cap", C.c_uint64)]
+
+
+class ServerDone(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int), ("result", Result),
+                ("queue_ms", C.c_double), ("service_ms", C.c_double), ("latency_ms", C.c_double),
+                ("footprint", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "stopped", "failed", "pending",
+                                          "running", "peak_running", "pool_bytes", "pool_used",
+                                          "pool_peak", "variants", "root_queues")]
 
 
 # Every function declared in include/beep.h with its ctypes signature.
@@ -70,6 +88,13 @@ SIGNATURES = {
                                     C.POINTER(C.c_void_p)]),
     "beep_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Result), C.POINTER(C.c_double)]),
     "beep_batch_free": (None, [C.c_void_p]),
+    "beep_server_opts_default": (None, [C.POINTER(ServerOpts)]),
+    "beep_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "beep_server_prepare": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "beep_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(C.c_uint64)]),
+    "beep_server_poll": (C.c_int, [C.c_void_p, C.POINTER(ServerDone), C.c_int]),
+    "beep_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "beep_server_free": (None, [C.c_void_p]),
     "beep_tuple_hash": (C.c_uint64, [u32p, C.c_int]),
     "beep_last_error": (C.c_char_p, []),
     "beep_version": (C.c_char_p, []),

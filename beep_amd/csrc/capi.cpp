@@ -27,6 +27,18 @@ void batch_free(Batch *b);
 struct beep_batch { beep::Batch *b; };
 
 namespace beep {
+struct Server;
+Server *server_create(const Graph *g, const beep_server_opts &o);
+void server_prepare(Server &s, const Plan &p);
+uint64_t server_submit(Server &s, const Plan &p, const beep_opts &o);
+bool server_poll(Server &s, beep_server_done &out, int timeout_ms);
+void server_stats(Server &s, beep_server_stats &out);
+void server_free(Server *s);
+}  // namespace beep
+
+struct beep_server { beep::Server *s; };
+
+namespace beep {
 
 void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res);
 void upload_prep(Prep &pp, int device, int skip_split);
@@ -208,23 +220,24 @@ int beep_prep_describe(const beep_prep *pp, beep_prep_desc *o)
     return guarded([&] {
         if (!pp || !o) throw Error(BEEP_EINVAL, "null argument");
         const auto &P = *pp->pp;
-        o->n = P.n;
-        o->m = P.m;
+        const auto &V = *P.var;
+        o->n = V.n;
+        o->m = V.m;
         o->n_roots = P.roots.size();
         o->q0_size = P.q0_size;
         o->boundary = P.boundary;
-        o->cutoff = P.cutoff;
-        o->max_degree = P.max_degree;
+        o->cutoff = V.cutoff;
+        o->max_degree = V.max_degree;
         o->max_root_degree = P.max_root_degree;
-        o->peel_rounds = P.peel_rounds;
-        o->asc = P.asc;
+        o->peel_rounds = V.peel_rounds;
+        o->asc = V.asc;
         o->alg_bytes = P.alg_bytes;
         o->n_hub_roots = P.n_hub;
         o->n_lds_batches = P.batches.size();
-        o->rowptr = P.rowptr.data();
-        o->col = P.col.data();
-        o->ocol = P.ocol.data();
-        o->nlow = P.nlow.data();
+        o->rowptr = V.rowptr.data();
+        o->col = V.col.data();
+        o->ocol = V.ocol.data();
+        o->nlow = V.nlow.data();
         o->roots = P.roots.data();
     });
 }
@@ -261,8 +274,10 @@ int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_resul
         if (opt.emit < BEEP_EMIT_NONE || opt.emit > BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "bad emit mode");
         if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
             throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
+        if (!(opt.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
         memset(out, 0, sizeof(*out));
         beep::count_on_device(*pp->pp, *p->p, opt, *out);
+        if (out->stopped) throw Error(BEEP_ETIMEDOUT, "deadline passed: the counts are partial");
     });
 }
 
@@ -298,6 +313,70 @@ void beep_batch_free(beep_batch *b)
     if (!b) return;
     beep::batch_free(b->b);
     delete b;
+}
+
+void beep_server_opts_default(beep_server_opts *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->lanes = 4;
+}
+
+int beep_server_create(const beep_graph *g, const beep_server_opts *o, beep_server **out)
+{
+    return guarded([&] {
+        if (!g || !out) throw Error(BEEP_EINVAL, "null argument");
+        beep_server_opts def;
+        beep_server_opts_default(&def);
+        *out = new beep_server{beep::server_create(g->g.get(), o ? *o : def)};
+    });
+}
+
+int beep_server_prepare(beep_server *s, const beep_plan *p)
+{
+    return guarded([&] {
+        if (!s || !p) throw Error(BEEP_EINVAL, "null argument");
+        beep::server_prepare(*s->s, *p->p);
+    });
+}
+
+int beep_server_submit(beep_server *s, const beep_plan *p, const beep_opts *o, uint64_t *ticket)
+{
+    return guarded([&] {
+        if (!s || !p || !ticket) throw Error(BEEP_EINVAL, "null argument");
+        beep_opts def;
+        beep_opts_default(&def);
+        const beep_opts &opt = o ? *o : def;
+        if (opt.emit < BEEP_EMIT_NONE || opt.emit > BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "bad emit mode");
+        if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
+            throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
+        if (!(opt.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
+        *ticket = beep::server_submit(*s->s, *p->p, opt);
+    });
+}
+
+int beep_server_poll(beep_server *s, beep_server_done *out, int timeout_ms)
+{
+    return guarded([&] {
+        if (!s || !out) throw Error(BEEP_EINVAL, "null argument");
+        if (!beep::server_poll(*s->s, *out, timeout_ms)) throw Error(BEEP_ETIMEDOUT, "no query finished in time");
+    });
+}
+
+int beep_server_stats_get(beep_server *s, beep_server_stats *out)
+{
+    return guarded([&] {
+        if (!s || !out) throw Error(BEEP_EINVAL, "null argument");
+        beep::server_stats(*s->s, *out);
+    });
+}
+
+void beep_server_free(beep_server *s)
+{
+    if (!s) return;
+    beep::server_free(s->s);
+    delete s;
 }
 
 uint64_t beep_tuple_hash(const uint32_t *t, int len)

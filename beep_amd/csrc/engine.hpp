@@ -1,0 +1,135 @@
+// engine.hpp -- device-side state of prepared queries and the launch
+// sequence, shared by the single-query engine, batches and the server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "internal.hpp"
+#include "sgm_kernels.hpp"
+
+namespace beep {
+
+#define HIP_CHECK(x)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess)                                                                   \
+            throw Error(BEEP_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+// Counter words of one launch sequence (u64): acc[4] | hub cursors[64] |
+// LDS cursors (32 x u32) | donation tail, head, slot tail, donated |
+// deadline | stop flag.
+constexpr int kCtrAcc = 0, kCtrHub = 4, kCtrLds = 68, kCtrQTail = 84, kCtrQHead = 85, kCtrSlot = 86,
+              kCtrDonated = 87, kCtrDeadline = 88, kCtrStop = 89, kCtrWords = 90;
+
+// HBM copy of a variant: oriented rows and per-vertex records.
+struct DevGraph {
+    int device = -1;
+    uint32_t *ocol = nullptr;
+    VInfo *vinfo = nullptr;
+    ~DevGraph();
+};
+
+struct HubRound {
+    uint32_t h0, h1;      // hub index range
+    uint64_t g0, g1;      // global row range (bitmap build)
+    uint64_t i0, i1;      // global item range (DFS)
+    uint64_t ewords;      // bitmap words used
+    uint64_t hslots;      // hash slots used
+};
+
+// Read-only device records of one prepared root queue.
+struct DevRoots {
+    RootRec *recs = nullptr;
+    uint32_t *batch_first = nullptr, *batch_count = nullptr;
+    uint32_t nbatches = 0;
+    HubRec *hubs = nullptr;
+    uint32_t *hub_of_row = nullptr, *hub_of_item = nullptr;
+    int skip_split = -1;  // plan property the hub items were built for
+    uint32_t nhubs = 0;
+    uint32_t wmax = 0;
+    std::vector<HubRound> rounds;
+    uint64_t arena_budget = 0;            // bytes the rounds were cut for
+    uint64_t arena_words = 0, arena_slots = 0;
+    uint64_t don_words = 0, don_records = 0;  // upper bounds of the donation ring
+    void release();
+};
+
+// Scratch sizes of one launch sequence.
+struct ScratchNeeds {
+    uint64_t arena_words = 0, hash_slots = 0, stk_words = 0;
+    uint64_t queue_cap = 0, slot_cap = 0, dbuf_words = 0;
+    uint64_t bytes() const;
+};
+
+// Scratch of one launch sequence, carved out of one device region.
+struct Scratch {
+    uint64_t *arena = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr;
+    uint64_t *stk = nullptr;
+    HubChunk *queue = nullptr;
+    uint64_t *slots = nullptr;
+    uint32_t *dbuf = nullptr;
+    unsigned long long *ctr = nullptr;
+    ScratchNeeds cap;
+};
+void carve(Scratch &s, char *base, const ScratchNeeds &n);
+
+// Streams and events of one launch sequence: hub rounds on `main`, LDS lane
+// classes round-robin on the side streams.
+struct Lane {
+    static constexpr int kMaxSide = 3;
+    int device = -1;
+    int nside = 0;
+    hipStream_t main = nullptr;
+    hipStream_t side[kMaxSide] = {};
+    hipEvent_t ev[4] = {};  // start, hubs done, all done, counters copied
+    hipEvent_t side_done[kMaxSide] = {};
+    void create(int device, int nside);
+    ~Lane();
+};
+
+// Single-query device state of a Prep: its records, an owned scratch
+// region and a lane.
+struct DevPrep {
+    int device = -1;
+    int cus = 0;
+    DevRoots roots;
+    char *region = nullptr;
+    uint64_t region_bytes = 0;
+    Scratch scratch;
+    Lane lane;
+    ~DevPrep();
+};
+
+// Everything one enumeration launch sequence needs.
+struct LaunchCfg {
+    KPlan kp{};
+    KGraph kg{};
+    KOut out{};
+    HubQueue hq{};  // q == nullptr: donation off
+    int hub_grid = 0;
+    int nlev = 1;
+    int lmin = 0;
+    bool donate = false;
+    uint64_t deadline_ticks = 0;
+};
+
+int device_cus(int device);
+uint64_t wall_clock_khz(int device);
+
+// Uploads the variant (once per variant) and the root records of pp, with hub
+// rounds cut to arena_budget bytes (0: the default budget).
+void upload(Prep &pp, int device, int skip_split, uint64_t arena_budget = 0);
+ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, int cus);
+LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const Scratch &s, int cus);
+// clear counters + deadline, LDS classes on the side streams and hub rounds on
+// main, join, counters -> host_ctr; records lane.ev[0..3].
+void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane,
+                   unsigned long long *host_ctr);
+void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t st);
+void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res);
+
+}  // namespace beep

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -115,16 +116,35 @@ struct LdsBatch {
     uint32_t count;   // number of roots
 };
 
-struct Prep {
+struct DevGraph;  // device copy of a variant (sgm_engine.hip)
+
+// Peeled and degree-oriented graph.  It depends on the query only through
+// (min_qdeg, asc), so every query with the same pair can share one variant
+// (and its copy in HBM).
+struct Variant {
     uint32_t n = 0;
     uint64_t m = 0;
-    int asc = 0, min_qdeg = 0, max_qdeg = 0, peel_rounds = 0;
-    uint32_t cutoff = 0, max_degree = 0, max_root_degree = 0;
+    int asc = 0, min_qdeg = 0, peel_rounds = 0;
+    uint32_t cutoff = 0, max_degree = 0;
     std::vector<uint64_t> rowptr;  // peeled
     std::vector<uint32_t> col;     // peeled, id ascending
     std::vector<uint32_t> ocol;    // oriented
     std::vector<uint32_t> nlow;    // #smaller keys
     std::vector<uint32_t> hlen;    // length of the bitmap-build scan range of each row
+    uint32_t deg(uint32_t v) const { return (uint32_t)(rowptr[v + 1] - rowptr[v]); }
+
+    std::mutex dev_mu;             // guards dev
+    DevGraph *dev = nullptr;
+    ~Variant();
+};
+
+std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc);
+
+// One query's root queue and work split over a variant.
+struct Prep {
+    std::shared_ptr<Variant> var;
+    int max_qdeg = 0;
+    uint32_t max_root_degree = 0;
     std::vector<uint32_t> roots;   // R
     uint64_t q0_size = 0, boundary = 0;
     uint64_t alg_bytes = 0;
@@ -146,7 +166,11 @@ struct Prep {
     ~Prep();
 };
 
+// Fresh variant + root queue.
 std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p);
+// Root queue over an existing variant (which must match the plan's
+// min_qdeg and asc).
+std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &v, const Plan &p);
 
 // LDS budget of the batch kernel (words / slots); shared with the kernels.
 struct LdsBudget {

@@ -95,18 +95,18 @@ int peel(const Graph &g, int min_qdeg, std::vector<uint64_t> &rp, std::vector<ui
 
 }  // namespace
 
-std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
+std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc_flag)
 {
-    auto pp = std::make_unique<Prep>();
-    pp->n = g.n;
-    pp->asc = p.asc;
-    pp->min_qdeg = p.min_qdeg;
-    pp->max_qdeg = p.max_qdeg;
-    pp->cutoff = g.cutoff;
-    pp->peel_rounds = peel(g, p.min_qdeg, pp->rowptr, pp->col);
+    auto vp = std::make_shared<Variant>();
+    Variant &V = *vp;
+    V.n = g.n;
+    V.asc = asc_flag;
+    V.min_qdeg = min_qdeg;
+    V.cutoff = g.cutoff;
+    V.peel_rounds = peel(g, min_qdeg, V.rowptr, V.col);
     const uint32_t n = g.n;
-    pp->m = pp->rowptr[n];
-    const auto &rp = pp->rowptr;
+    V.m = V.rowptr[n];
+    const auto &rp = V.rowptr;
     std::vector<uint32_t> deg(n);
     parallel_for(n, [&](size_t b, size_t e) {
         for (size_t v = b; v < e; v++) deg[v] = (uint32_t)(rp[v + 1] - rp[v]);
@@ -114,35 +114,35 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
     // Orientation: key(v) = (deg(v), v).  asc rows ascend by key; desc rows
     // descend by degree with ids ascending (the stable sorts of :1305-1331 on an
     // id-ascending input).  nlow(u) = #neighbours with a smaller key.
-    pp->ocol.resize(pp->m);
-    pp->nlow.assign(n, 0);
-    pp->hlen.assign(n, 0);
-    const bool asc = p.asc != 0;
+    V.ocol.resize(V.m);
+    V.nlow.assign(n, 0);
+    V.hlen.assign(n, 0);
+    const bool asc = asc_flag != 0;
     parallel_for(n, [&](size_t b, size_t e) {
         for (size_t u = b; u < e; u++) {
             const uint64_t rb = rp[u], re = rp[u + 1];
             uint32_t du = deg[u], lo = 0;
             for (uint64_t x = rb; x < re; x++) {
-                uint32_t w = pp->col[x];
+                uint32_t w = V.col[x];
                 if (deg[w] < du || (deg[w] == du && w < (uint32_t)u)) lo++;
-                pp->ocol[x] = w;
+                V.ocol[x] = w;
             }
-            pp->nlow[u] = lo;
+            V.nlow[u] = lo;
             // Bitmap-build scan range: every adjacent pair {v, w} is found from
             // exactly one endpoint.  asc rows: the suffix of larger keys.  desc
             // rows (degree descending, ids ascending): the prefix of larger
             // degrees followed by equal degrees with smaller ids.
             if (asc) {
-                pp->hlen[u] = du - lo;
+                V.hlen[u] = du - lo;
             } else {
                 uint32_t c = 0;
                 for (uint64_t x = rb; x < re; x++) {
-                    uint32_t w = pp->col[x];
+                    uint32_t w = V.col[x];
                     if (deg[w] > du || (deg[w] == du && w < (uint32_t)u)) c++;
                 }
-                pp->hlen[u] = c;
+                V.hlen[u] = c;
             }
-            uint32_t *row = pp->ocol.data() + rb;
+            uint32_t *row = V.ocol.data() + rb;
             if (asc)
                 std::sort(row, row + (re - rb), [&](uint32_t a, uint32_t c) {
                     return deg[a] != deg[c] ? deg[a] < deg[c] : a < c;
@@ -155,36 +155,56 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
     }, 4096);
     uint32_t maxd = 0;
     for (uint32_t v = 0; v < n; v++) maxd = std::max(maxd, deg[v]);
-    pp->max_degree = maxd;
+    V.max_degree = maxd;
+    return vp;
+}
+
+std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
+{
+    return prepare_on(make_variant(g, p.min_qdeg, p.asc), p);
+}
+
+std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan &p)
+{
+    if (!vp || vp->min_qdeg != p.min_qdeg || vp->asc != p.asc)
+        throw Error(BEEP_EINVAL, "variant does not match the plan");
+    const Variant &V = *vp;
+    auto pp = std::make_unique<Prep>();
+    pp->var = vp;
+    pp->max_qdeg = p.max_qdeg;
+    const uint32_t n = V.n, maxd = V.max_degree;
+    const auto &rp = V.rowptr;
+    const bool asc = V.asc != 0;
+    auto deg = [&](uint32_t v) { return (uint32_t)(rp[v + 1] - rp[v]); };
     // Roots: Q0 = {max_qdeg <= d < 100 cutoff} ordered by degree descending
     // with ids ascending; boundary = #{d > cutoff}; the head of Q0 is skipped
     // when ascending and boundary >= 1 (work_list_head1 starts at 1).
-    const uint32_t hi = g.cutoff * 100u;
+    const uint32_t hi = V.cutoff * 100u;
     const uint32_t lo = (uint32_t)p.max_qdeg;
     std::vector<uint64_t> bucket((size_t)maxd + 2, 0);
     uint64_t nq = 0;
     for (uint32_t v = 0; v < n; v++)
-        if (deg[v] >= lo && deg[v] < hi) { bucket[maxd - deg[v]]++; nq++; }
+        if (deg(v) >= lo && deg(v) < hi) { bucket[maxd - deg(v)]++; nq++; }
     uint64_t s = 0;
     for (auto &c : bucket) { uint64_t t = c; c = s; s += t; }
     std::vector<uint32_t> q0(nq);
     for (uint32_t v = 0; v < n; v++)
-        if (deg[v] >= lo && deg[v] < hi) q0[bucket[maxd - deg[v]]++] = v;
+        if (deg(v) >= lo && deg(v) < hi) q0[bucket[maxd - deg(v)]++] = v;
     uint64_t bnd = 0;
-    while (bnd < nq && deg[q0[bnd]] > g.cutoff) bnd++;
+    while (bnd < nq && deg(q0[bnd]) > V.cutoff) bnd++;
     const uint64_t skip = (asc && bnd >= 1) ? 1 : 0;
     pp->roots.assign(q0.begin() + (ptrdiff_t)skip, q0.end());
     pp->q0_size = nq;
     pp->boundary = bnd;
-    pp->max_root_degree = pp->roots.empty() ? 0 : deg[pp->roots[0]];
+    pp->max_root_degree = pp->roots.empty() ? 0 : deg(pp->roots[0]);
     // B_alg = sum over roots of 4(d(u)+2) + sum over neighbours 4(d(v)+2).
     std::atomic<uint64_t> bytes{0};
     parallel_for(pp->roots.size(), [&](size_t b, size_t e) {
         uint64_t acc = 0;
         for (size_t r = b; r < e; r++) {
             uint32_t u = pp->roots[r];
-            acc += 4ull * (deg[u] + 2);
-            for (uint64_t x = rp[u]; x < rp[u + 1]; x++) acc += 4ull * (deg[pp->col[x]] + 2);
+            acc += 4ull * (deg(u) + 2);
+            for (uint64_t x = rp[u]; x < rp[u + 1]; x++) acc += 4ull * (deg(V.col[x]) + 2);
         }
         bytes.fetch_add(acc);
     }, 1024);
@@ -197,27 +217,29 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
     pp->rows_above_split = p.rows_above_split;
     pp->uses_bitmap = p.uses_bitmap;
     {
-        std::vector<uint32_t> idx;
-        idx.reserve(pp->roots.size());
-        for (uint32_t r = 0; r < (uint32_t)pp->roots.size(); r++) {
-            uint32_t u = pp->roots[r];
-            uint32_t len = pp->reduced ? deg[u] - pp->nlow[u] : deg[u];
-            if (len) idx.push_back(r);
-        }
         auto len_of = [&](uint32_t r) {
             uint32_t u = pp->roots[r];
-            return pp->reduced ? deg[u] - pp->nlow[u] : deg[u];
+            return pp->reduced ? deg(u) - V.nlow[u] : deg(u);
         };
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return len_of(a) > len_of(b); });
+        std::vector<uint32_t> idx;
+        idx.reserve(pp->roots.size());
+        for (uint32_t r = 0; r < (uint32_t)pp->roots.size(); r++)
+            if (len_of(r)) idx.push_back(r);
+        // roots are in degree-descending order already; only reduced lengths
+        // need a (stable) sort
+        if (pp->reduced)
+            std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return len_of(a) > len_of(b); });
         pp->wu.resize(idx.size());
         pp->woff.resize(idx.size());
         pp->wlen.resize(idx.size());
-        for (size_t i = 0; i < idx.size(); i++) {
-            uint32_t u = pp->roots[idx[i]];
-            pp->wu[i] = u;
-            pp->woff[i] = pp->reduced ? pp->nlow[u] : 0u;
-            pp->wlen[i] = len_of(idx[i]);
-        }
+        parallel_for(idx.size(), [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; i++) {
+                uint32_t u = pp->roots[idx[i]];
+                pp->wu[i] = u;
+                pp->woff[i] = pp->reduced ? V.nlow[u] : 0u;
+                pp->wlen[i] = len_of(idx[i]);
+            }
+        });
     }
     build_tiers(*pp);
     return pp;

@@ -5,19 +5,18 @@
 // copies __constant__ tables, sizes a grid by occupancy, allocates and memsets
 // per-block global scratch, launches the persistent kernel, adds the device
 // counter into a managed counter and prints it.  Here the prepared graph is
-// uploaded once (beep_prep_upload) and stays in HBM; beep_count launches, on
-// one stream,
-//   per hub round:   arena clear -> hub_hash -> hub_encode -> hub_dfs
-//   per lane class:  lds_batch_kernel<P>
-// and reads back four counters.
+// uploaded once (beep_prep_upload) and stays in HBM; beep_count launches
+//   on one stream, per hub round:  arena clear -> hub_hash -> hub_encode -> hub_dfs
+//   on side streams, per lane class: lds_batch_kernel<P>
+// and reads back the counters.  Every buffer of the sequence is carved from
+// one scratch region sized before anything is enqueued.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
 
-#include "internal.hpp"
-#include "sgm_kernels.hpp"
+#include "engine.hpp"
 
 namespace beep {
 
@@ -26,6 +25,7 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
                               uint32_t b0, uint32_t b1, unsigned int *cursor, uint32_t ewords,
                               const KOut &out, int *grid_out);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
+hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks, hipStream_t st);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
@@ -36,13 +36,6 @@ hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
                           const HubQueue &hq, int mode);
 int hub_dfs_occupancy(int nlev, uint32_t wmax);
-
-#define HIP_CHECK(x)                                                                            \
-    do {                                                                                        \
-        hipError_t e_ = (x);                                                                    \
-        if (e_ != hipSuccess)                                                                   \
-            throw Error(BEEP_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));         \
-    } while (0)
 
 template <class T>
 static T *dev_alloc(size_t n)
@@ -61,105 +54,123 @@ static T *dev_upload(const std::vector<T> &v)
     return p;
 }
 
-constexpr int kCtrAcc = 0, kCtrHub = 4, kCtrLds = 68, kCtrQTail = 84, kCtrQHead = 85, kCtrSlot = 86,
-              kCtrDonated = 87, kCtrWords = 88;
+// ------------------------------------------------------------ residency
 
-struct HubRound {
-    uint32_t h0, h1;      // hub index range
-    uint64_t g0, g1;      // global row range (bitmap build)
-    uint64_t i0, i1;      // global item range (DFS)
-    uint64_t ewords;      // bitmap words used
-    uint64_t hslots;      // hash slots used
-};
+DevGraph::~DevGraph()
+{
+    if (device >= 0) (void)hipSetDevice(device);
+    if (ocol) (void)hipFree(ocol);
+    if (vinfo) (void)hipFree(vinfo);
+}
 
-struct DevPrep {
-    int device = -1;
-    int cus = 0;
-    uint32_t *ocol = nullptr;
-    VInfo *vinfo = nullptr;
-    // LDS tier
-    RootRec *recs = nullptr;
-    uint32_t *batch_first = nullptr, *batch_count = nullptr;
-    uint32_t nbatches = 0;
-    // hub tier
-    HubRec *hubs = nullptr;
-    uint32_t *hub_of_row = nullptr, *hub_of_item = nullptr;
-    int skip_split = -1;  // plan property the hub items were built for
-    uint32_t nhubs = 0;
-    uint32_t wmax = 0;
-    std::vector<HubRound> rounds;
-    uint64_t arena_words = 0, arena_slots = 0;
-    uint64_t *arena = nullptr;
-    uint32_t *keys = nullptr, *vals = nullptr;
-    uint64_t *stk = nullptr;
-    size_t stk_words = 0;
-    // donation queue of the hub DFS
-    HubChunk *queue = nullptr;
-    uint64_t *slots = nullptr;
-    uint64_t queue_cap = 0, slot_cap = 0;
-    // counters: acc[4] | hub cursors[64] | lds cursors[16] | queue tail, head,
-    // slot tail, donated
-    unsigned long long *ctr = nullptr;
-    hipStream_t stream = nullptr;
-    static constexpr int kSide = 3;      // streams of the LDS lane classes
-    hipStream_t side[kSide] = {};
-    hipEvent_t ev[4] = {};
-    hipEvent_t side_done[kSide] = {};
+Variant::~Variant() { delete dev; }
 
-    ~DevPrep()
-    {
-        int cur = 0;
-        if (hipGetDevice(&cur) != hipSuccess) cur = device;
-        if (device >= 0) (void)hipSetDevice(device);
-        for (void *p : {(void *)ocol, (void *)vinfo, (void *)recs, (void *)batch_first,
-                        (void *)batch_count, (void *)hubs, (void *)hub_of_row, (void *)hub_of_item, (void *)arena,
-                        (void *)keys, (void *)vals, (void *)stk, (void *)ctr, (void *)queue, (void *)slots})
-            if (p) (void)hipFree(p);
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (auto &e : side_done)
-            if (e) (void)hipEventDestroy(e);
-        for (auto &s : side)
-            if (s) (void)hipStreamDestroy(s);
-        if (stream) (void)hipStreamDestroy(stream);
-        if (device >= 0) (void)hipSetDevice(cur);
-    }
-};
+void DevRoots::release()
+{
+    for (void *p : {(void *)recs, (void *)batch_first, (void *)batch_count, (void *)hubs, (void *)hub_of_row,
+                    (void *)hub_of_item})
+        if (p) (void)hipFree(p);
+    recs = nullptr;
+    batch_first = batch_count = hub_of_row = hub_of_item = nullptr;
+    hubs = nullptr;
+}
+
+void Lane::create(int dev, int ns)
+{
+    if (main) return;
+    device = dev;
+    nside = std::max(1, std::min(ns, kMaxSide));
+    HIP_CHECK(hipStreamCreateWithFlags(&main, hipStreamNonBlocking));
+    for (int i = 0; i < nside; i++) HIP_CHECK(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking));
+    for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+    for (int i = 0; i < nside; i++) HIP_CHECK(hipEventCreate(&side_done[i]));
+}
+
+Lane::~Lane()
+{
+    if (device >= 0) (void)hipSetDevice(device);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : side_done)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &s : side)
+        if (s) (void)hipStreamDestroy(s);
+    if (main) (void)hipStreamDestroy(main);
+}
+
+DevPrep::~DevPrep()
+{
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = device;
+    if (device >= 0) (void)hipSetDevice(device);
+    roots.release();
+    if (region) (void)hipFree(region);
+    if (device >= 0) (void)hipSetDevice(cur);
+}
 
 Prep::~Prep() { delete dev; }
 
-static void upload(Prep &pp, int device, int p_skip_split)
+int device_cus(int device)
 {
-    if (pp.dev && pp.dev->device == device && pp.dev->skip_split == p_skip_split) return;
-    delete pp.dev;
-    pp.dev = nullptr;
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    return prop.multiProcessorCount;
+}
+
+uint64_t wall_clock_khz(int device)
+{
+    int khz = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    return khz > 0 ? (uint64_t)khz : 100000ull;
+}
+
+static void upload_variant(Variant &V, int device)
+{
+    std::lock_guard<std::mutex> lk(V.dev_mu);
+    if (V.dev) {
+        if (V.dev->device != device) throw Error(BEEP_EINVAL, "the prepared graph is resident on another device");
+        return;
+    }
+    auto *dg = new DevGraph();
+    try {
+        dg->device = device;
+        dg->ocol = dev_upload(V.ocol);
+        std::vector<VInfo> vi(V.n);
+        parallel_for(V.n, [&](size_t b, size_t e2) {
+            for (size_t v = b; v < e2; v++) {
+                vi[v].hbeg = V.rowptr[v] + (V.asc ? V.nlow[v] : 0u);
+                vi[v].hlen = V.hlen[v];
+                vi[v].deg = (uint32_t)(V.rowptr[v + 1] - V.rowptr[v]);
+            }
+        });
+        dg->vinfo = dev_upload(vi);
+    } catch (...) {
+        delete dg;
+        throw;
+    }
+    V.dev = dg;
+}
+
+void upload(Prep &pp, int device, int p_skip_split, uint64_t arena_budget)
+{
     int ndev = 0;
     HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw Error(BEEP_EDEVICE, "no such HIP device");
     HIP_CHECK(hipSetDevice(device));
-    auto *d = new DevPrep();
-    pp.dev = d;
+    upload_variant(*pp.var, device);
+    if (pp.dev && pp.dev->device == device && pp.dev->roots.skip_split == p_skip_split &&
+        (arena_budget == 0 || pp.dev->roots.arena_budget == arena_budget))
+        return;
+    delete pp.dev;
+    pp.dev = nullptr;
+    const Variant &V = *pp.var;
+    // built aside and published at the end, so a failure leaves no half-built state
+    std::unique_ptr<DevPrep> dp(new DevPrep());
+    DevPrep *d = dp.get();
     d->device = device;
-    d->skip_split = p_skip_split;
-    hipDeviceProp_t prop;
-    HIP_CHECK(hipGetDeviceProperties(&prop, device));
-    d->cus = prop.multiProcessorCount;
-    HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
-    for (auto &s : d->side) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    for (auto &e : d->ev) HIP_CHECK(hipEventCreate(&e));
-    for (auto &e : d->side_done) HIP_CHECK(hipEventCreate(&e));
-    d->ocol = dev_upload(pp.ocol);
-    {
-        std::vector<VInfo> vi(pp.n);
-        parallel_for(pp.n, [&](size_t b, size_t e2) {
-            for (size_t v = b; v < e2; v++) {
-                vi[v].hbeg = pp.rowptr[v] + (pp.asc ? pp.nlow[v] : 0u);
-                vi[v].hlen = pp.hlen[v];
-                vi[v].deg = (uint32_t)(pp.rowptr[v + 1] - pp.rowptr[v]);
-            }
-        });
-        d->vinfo = dev_upload(vi);
-    }
+    d->cus = device_cus(device);
+    DevRoots &R = d->roots;
+    R.skip_split = p_skip_split;
 
     // LDS batches: per-root records with batch-relative offsets.
     {
@@ -173,10 +184,10 @@ static void upload(Prep &pp, int device, int p_skip_split)
                 const uint32_t wi = b.first + i;
                 const uint32_t u = pp.wu[wi];
                 RootRec r{};
-                r.rbeg = pp.rowptr[u] + pp.woff[wi];
+                r.rbeg = V.rowptr[u] + pp.woff[wi];
                 r.u = u;
                 r.d = pp.wlen[wi];
-                r.split = pp.reduced ? 0u : pp.nlow[u];
+                r.split = pp.reduced ? 0u : V.nlow[u];
                 r.doff = doff;
                 r.eoff = eoff;
                 r.hoff = hoff;
@@ -186,10 +197,10 @@ static void upload(Prep &pp, int device, int p_skip_split)
                 recs.push_back(r);
             }
         }
-        d->recs = dev_upload(recs);
-        d->batch_first = dev_upload(bf);
-        d->batch_count = dev_upload(bc);
-        d->nbatches = (uint32_t)bf.size();
+        R.recs = dev_upload(recs);
+        R.batch_first = dev_upload(bf);
+        R.batch_count = dev_upload(bc);
+        R.nbatches = (uint32_t)bf.size();
     }
 
     // Hubs: records, row -> hub map and rounds that fit the arena.
@@ -200,10 +211,10 @@ static void upload(Prep &pp, int device, int p_skip_split)
         for (uint32_t h = 0; h < pp.n_hub; h++) {
             const uint32_t u = pp.wu[h];
             HubRec r{};
-            r.rbeg = pp.rowptr[u] + pp.woff[h];
+            r.rbeg = V.rowptr[u] + pp.woff[h];
             r.u = u;
             r.d = pp.wlen[h];
-            r.split = pp.reduced ? 0u : pp.nlow[u];
+            r.split = pp.reduced ? 0u : V.nlow[u];
             r.hslots = hash_slots_for(r.d);
             r.joff = joff;
             r.ioff = ioff;
@@ -214,20 +225,27 @@ static void upload(Prep &pp, int device, int p_skip_split)
             ioff += r.d - r.jbeg;
             for (uint32_t j = 0; j < r.d; j++) hor.push_back(h);
             for (uint32_t j = r.jbeg; j < r.d; j++) hoi.push_back(h);
-            d->wmax = std::max<uint32_t>(d->wmax, (r.d + 63) / 64);
+            const uint64_t W = (r.d + 63) / 64;
+            R.wmax = std::max<uint32_t>(R.wmax, (uint32_t)W);
+            // donation: at most one slot of W words and ceil(d/16)+1 records per item
+            R.don_words += (uint64_t)(r.d - r.jbeg) * W;
+            R.don_records += (uint64_t)(r.d - r.jbeg) * ((r.d + 15) / 16 + 1);
             hubs.push_back(r);
         }
-        size_t freeb = 0, totalb = 0;
-        HIP_CHECK(hipMemGetInfo(&freeb, &totalb));
-        const uint64_t budget_bytes = std::min<uint64_t>((uint64_t)(freeb * 0.35), 48ull << 30);
+        if (arena_budget == 0) {
+            size_t freeb = 0, totalb = 0;
+            HIP_CHECK(hipMemGetInfo(&freeb, &totalb));
+            arena_budget = std::min<uint64_t>((uint64_t)(freeb * 0.35), 48ull << 30);
+        }
+        R.arena_budget = arena_budget;
         uint64_t ew = 0, hsl = 0;
         uint32_t h0 = 0;
         for (uint32_t h = 0; h < (uint32_t)hubs.size(); h++) {
             // plans that never read a bitmap row need neither bitmap nor hash
             uint64_t w = pp.uses_bitmap ? (uint64_t)(hubs[h].d - hubs[h].rlo) * ((hubs[h].d + 63) / 64) : 0;
             uint64_t s = pp.uses_bitmap ? hubs[h].hslots : 0;
-            if (h > h0 && (ew + w) * 8 + (hsl + s) * 8 > budget_bytes) {
-                d->rounds.push_back({h0, h, hubs[h0].joff, hubs[h].joff, hubs[h0].ioff, hubs[h].ioff, ew, hsl});
+            if (h > h0 && (ew + w) * 8 + (hsl + s) * 8 > arena_budget) {
+                R.rounds.push_back({h0, h, hubs[h0].joff, hubs[h].joff, hubs[h0].ioff, hubs[h].ioff, ew, hsl});
                 h0 = h;
                 ew = hsl = 0;
             }
@@ -235,26 +253,78 @@ static void upload(Prep &pp, int device, int p_skip_split)
             hubs[h].hoff = hsl;
             ew += w;
             hsl += s;
-            if ((ew * 8 + hsl * 8) > budget_bytes)
+            if ((ew * 8 + hsl * 8) > arena_budget)
                 throw Error(BEEP_ENOMEM, "hub root bitmap exceeds the HBM arena budget");
         }
         if (!hubs.empty())
-            d->rounds.push_back({h0, (uint32_t)hubs.size(), hubs[h0].joff, joff, hubs[h0].ioff, ioff, ew, hsl});
-        for (const auto &r : d->rounds) {
-            d->arena_words = std::max(d->arena_words, r.ewords);
-            d->arena_slots = std::max(d->arena_slots, r.hslots);
+            R.rounds.push_back({h0, (uint32_t)hubs.size(), hubs[h0].joff, joff, hubs[h0].ioff, ioff, ew, hsl});
+        for (const auto &r : R.rounds) {
+            R.arena_words = std::max(R.arena_words, r.ewords);
+            R.arena_slots = std::max(R.arena_slots, r.hslots);
         }
-        d->hubs = dev_upload(hubs);
-        d->hub_of_row = dev_upload(hor);
-        d->hub_of_item = dev_upload(hoi);
-        d->nhubs = (uint32_t)hubs.size();
-        if (d->nhubs) {
-            d->arena = dev_alloc<uint64_t>(d->arena_words);
-            d->keys = dev_alloc<uint32_t>(d->arena_slots);
-            d->vals = dev_alloc<uint32_t>(d->arena_slots);
-        }
+        R.hubs = dev_upload(hubs);
+        R.hub_of_row = dev_upload(hor);
+        R.hub_of_item = dev_upload(hoi);
+        R.nhubs = (uint32_t)hubs.size();
     }
-    d->ctr = dev_alloc<unsigned long long>(kCtrWords);
+    pp.dev = dp.release();
+}
+
+// ------------------------------------------------------------ scratch
+
+static uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
+
+uint64_t ScratchNeeds::bytes() const
+{
+    return align256(kCtrWords * 8) + align256(arena_words * 8) + 2 * align256(hash_slots * 4) +
+           align256(stk_words * 8) + align256(queue_cap * sizeof(HubChunk)) + align256(slot_cap * 8) +
+           align256(dbuf_words * 4);
+}
+
+void carve(Scratch &s, char *base, const ScratchNeeds &n)
+{
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        char *p = base + off;
+        off += align256(bytes);
+        return p;
+    };
+    s.ctr = (unsigned long long *)take(kCtrWords * 8);
+    s.arena = n.arena_words ? (uint64_t *)take(n.arena_words * 8) : nullptr;
+    s.keys = n.hash_slots ? (uint32_t *)take(n.hash_slots * 4) : nullptr;
+    s.vals = n.hash_slots ? (uint32_t *)take(n.hash_slots * 4) : nullptr;
+    s.stk = n.stk_words ? (uint64_t *)take(n.stk_words * 8) : nullptr;
+    s.queue = n.queue_cap ? (HubChunk *)take(n.queue_cap * sizeof(HubChunk)) : nullptr;
+    s.slots = n.slot_cap ? (uint64_t *)take(n.slot_cap * 8) : nullptr;
+    s.dbuf = n.dbuf_words ? (uint32_t *)take(n.dbuf_words * 4) : nullptr;
+    s.cap = n;
+}
+
+static int hub_grid_of(const DevRoots &R, int nlev, int cus)
+{
+    return hub_dfs_occupancy(nlev, std::max<uint32_t>(R.wmax, 1)) * cus;
+}
+
+ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, int cus)
+{
+    const DevRoots &R = pp.dev->roots;
+    ScratchNeeds n;
+    if (R.nhubs && pp.uses_bitmap) {
+        n.arena_words = R.arena_words;
+        n.hash_slots = R.arena_slots;
+    }
+    const int nlev = p.L > 3 ? p.L - 3 : 1;
+    if (R.nhubs) {
+        // per wave: nlev stack rows + the row of x_1 (used when they do not fit LDS)
+        n.stk_words = (uint64_t)hub_grid_of(R, nlev, cus) * (kBlock / 64) * (nlev + 1) *
+                      std::max<uint32_t>(R.wmax, 1);
+    }
+    if (o.donate && R.nhubs && p.L >= 4) {
+        n.queue_cap = std::max<uint64_t>(1, std::min<uint64_t>(1ull << 22, R.don_records));
+        n.slot_cap = std::max<uint64_t>(1, std::min<uint64_t>(1ull << 26, R.don_words));
+    }
+    if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) n.dbuf_words = o.match_cap * (uint64_t)p.L;
+    return n;
 }
 
 static KPlan make_kplan(const Plan &p, int emit)
@@ -271,120 +341,120 @@ static KPlan make_kplan(const Plan &p, int emit)
     return k;
 }
 
-// Everything one enumeration launch sequence needs; built (and buffers
-// allocated) before anything is enqueued, so the sequence can be captured.
-struct LaunchCfg {
-    KPlan kp;
-    KGraph kg;
-    KOut out;
-    HubQueue hq;
-    int hub_grid = 0;
-    int nlev = 1;
-    int lmin = 0;
-    bool donate = false;
-};
-
-static LaunchCfg prepare_launch(Prep &pp, const Plan &p, const beep_opts &o, uint32_t *dbuf)
+LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const Scratch &s, int cus)
 {
-    upload(pp, o.device, p.skip_split);
-    DevPrep &d = *pp.dev;
-    HIP_CHECK(hipSetDevice(d.device));
-    if (p.min_qdeg != pp.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != pp.asc ||
-        p.above_split != pp.reduced || p.rows_above_split != pp.rows_above_split)
+    const Variant &V = *pp.var;
+    if (p.min_qdeg != V.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != V.asc ||
+        p.above_split != pp.reduced || p.rows_above_split != pp.rows_above_split ||
+        p.uses_bitmap != pp.uses_bitmap || p.skip_split != pp.dev->roots.skip_split)
         throw Error(BEEP_EINVAL, "plan does not match the prepared graph");
-    LaunchCfg c;
-    c.out = KOut{d.ctr + kCtrAcc, dbuf, (o.emit == BEEP_EMIT_BUFFER) ? o.match_cap : 0};
+    const DevRoots &R = pp.dev->roots;
+    LaunchCfg c{};
+    const bool buf = o.emit == BEEP_EMIT_BUFFER && o.match_cap && s.dbuf;
+    c.out = KOut{s.ctr + kCtrAcc, buf ? s.dbuf : nullptr, buf ? o.match_cap : 0, s.ctr + kCtrDeadline,
+                 s.ctr + kCtrStop};
     c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
-    c.kg = KGraph{d.ocol, d.vinfo};
+    c.kg = KGraph{V.dev->ocol, V.dev->vinfo};
     c.nlev = p.L > 3 ? p.L - 3 : 1;
-    c.hub_grid = hub_dfs_occupancy(c.nlev, std::max<uint32_t>(d.wmax, 1)) * d.cus;
-    if (d.nhubs) {
-        // per wave: nlev stack rows + the row of x_1 (used when they do not fit LDS)
-        size_t need = (size_t)c.hub_grid * (kBlock / 64) * (c.nlev + 1) * std::max<uint32_t>(d.wmax, 1);
-        if (need > d.stk_words) {
-            if (d.stk) HIP_CHECK(hipFree(d.stk));
-            d.stk = dev_alloc<uint64_t>(need);
-            d.stk_words = need;
-        }
-    }
-    // donation queue (hub items with |C_2| > donate_min), allocated on first use
-    c.donate = o.donate && d.nhubs && p.L >= 4;
-    if (c.donate && !d.queue) {
-        d.queue_cap = 1ull << 22;
-        d.slot_cap = 1ull << 26;
-        d.queue = dev_alloc<HubChunk>(d.queue_cap);
-        d.slots = dev_alloc<uint64_t>(d.slot_cap);
-    }
+    c.hub_grid = hub_grid_of(R, c.nlev, cus);
+    // donation ring (hub items with |C_2| > donate_min)
+    c.donate = o.donate && R.nhubs && p.L >= 4 && s.queue && s.slots;
     if (c.donate) {
-        c.hq.q = d.queue;
-        c.hq.cap = d.queue_cap;
-        c.hq.slots = d.slots;
-        c.hq.slot_cap = d.slot_cap;
-        c.hq.tail = d.ctr + kCtrQTail;
-        c.hq.head = d.ctr + kCtrQHead;
-        c.hq.slot_tail = d.ctr + kCtrSlot;
-        c.hq.donated = d.ctr + kCtrDonated;
+        c.hq.q = s.queue;
+        c.hq.cap = s.cap.queue_cap;
+        c.hq.slots = s.slots;
+        c.hq.slot_cap = s.cap.slot_cap;
+        c.hq.tail = s.ctr + kCtrQTail;
+        c.hq.head = s.ctr + kCtrQHead;
+        c.hq.slot_tail = s.ctr + kCtrSlot;
+        c.hq.donated = s.ctr + kCtrDonated;
         c.hq.split_min = o.donate_min ? o.donate_min : 64;
         c.hq.chunk = c.hq.split_min;
     }
     c.lmin = 0;
     if (o.lanes > 0)
         while ((1 << c.lmin) < o.lanes && c.lmin < 6) c.lmin++;
+    c.deadline_ticks = o.deadline_ms > 0 ? (uint64_t)(o.deadline_ms * (double)wall_clock_khz(pp.dev->device)) : 0;
+    if (o.deadline_ms > 0 && c.deadline_ticks == 0) c.deadline_ticks = 1;
     return c;
 }
 
+// ------------------------------------------------------------ launch sequence
+
 // Hub rounds on one stream: clear, hash, bitmap rows, DFS + donation drain.
-static void enqueue_hubs(Prep &pp, const LaunchCfg &c, hipStream_t st)
+void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t st)
 {
-    DevPrep &d = *pp.dev;
-    for (size_t ri = 0; ri < d.rounds.size(); ri++) {
-        const HubRound &r = d.rounds[ri];
+    const DevRoots &R = pp.dev->roots;
+    const uint32_t wmax = std::max<uint32_t>(R.wmax, 1);
+    const int cus = pp.dev->cus;
+    for (size_t ri = 0; ri < R.rounds.size(); ri++) {
+        const HubRound &r = R.rounds[ri];
         const uint64_t rows = r.g1 - r.g0;
         if (pp.uses_bitmap) {
-            HIP_CHECK(launch_fill_zero(d.arena, r.ewords * 8, st));
-            HIP_CHECK(launch_fill_zero(d.keys, r.hslots * 4, st));
-            int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)d.cus * 16);
-            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys, d.vals));
-            int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)d.cus * 16);
-            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, d.hubs, d.hub_of_row, r.g0, r.g1, d.keys,
-                                        d.vals, d.arena));
+            HIP_CHECK(launch_fill_zero(s.arena, r.ewords * 8, st));
+            HIP_CHECK(launch_fill_zero(s.keys, r.hslots * 4, st));
+            int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)cus * 16);
+            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.keys, s.vals));
+            int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
+            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.keys,
+                                        s.vals, s.arena));
         }
-        unsigned long long *cur = d.ctr + kCtrHub + (ri % 64);
+        unsigned long long *cur = s.ctr + kCtrHub + (ri % 64);
         if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, 8, st));
         const uint64_t items = r.i1 - r.i0;
         if (items) {
-            if (c.donate && ri > 0) HIP_CHECK(launch_fill_zero(d.ctr + kCtrQTail, 3 * 8, st));
+            if (c.donate && ri > 0) HIP_CHECK(launch_fill_zero(s.ctr + kCtrQTail, 3 * 8, st));
             int g3 = (int)std::min<uint64_t>((items + 3) / 4, (uint64_t)c.hub_grid);
-            HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, c.kg, c.kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur,
-                                     d.arena, d.stk, std::max<uint32_t>(d.wmax, 1), c.out, c.hq, 0));
+            HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, c.kg, c.kp, R.hubs, R.hub_of_item, r.i0, r.i1, cur,
+                                     s.arena, s.stk, wmax, c.out, c.hq, 0));
             if (c.donate)
-                HIP_CHECK(launch_hub_dfs(c.hub_grid, st, c.kg, c.kp, d.hubs, d.hub_of_item, r.i0, r.i1, cur,
-                                         d.arena, d.stk, std::max<uint32_t>(d.wmax, 1), c.out, c.hq, 1));
+                HIP_CHECK(launch_hub_dfs(c.hub_grid, st, c.kg, c.kp, R.hubs, R.hub_of_item, r.i0, r.i1, cur,
+                                         s.arena, s.stk, wmax, c.out, c.hq, 1));
         }
     }
 }
 
 // LDS lane classes, one launch each, round-robin over the given streams.
-static void enqueue_lds_classes(Prep &pp, const LaunchCfg &c, hipStream_t *streams, int nst)
+static void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams,
+                                int nst)
 {
-    DevPrep &d = *pp.dev;
+    const DevRoots &R = pp.dev->roots;
     uint32_t b = 0;
     int cls = 0;
-    while (b < d.nbatches) {
+    while (b < R.nbatches) {
         int lg = std::max<int>(pp.batch_lanes_log2[b], c.lmin);
         uint32_t e = b + 1;
-        while (e < d.nbatches && std::max<int>(pp.batch_lanes_log2[e], c.lmin) == lg) e++;
+        while (e < R.nbatches && std::max<int>(pp.batch_lanes_log2[e], c.lmin) == lg) e++;
         uint32_t ewords = std::min<uint32_t>(LdsBudget::kMaxEWords, LdsBudget::kMaxSumDeg << std::min(lg, 3));
         if (cls >= 32) throw Error(BEEP_ERANGE, "too many lane classes");
-        unsigned int *cur = (unsigned int *)(d.ctr + kCtrLds) + cls;
-        HIP_CHECK(launch_lds_batches(lg, d.cus, streams[cls % nst], c.kg, c.kp, d.recs, d.batch_first,
-                                     d.batch_count, b, e, cur, ewords, c.out, nullptr));
+        unsigned int *cur = (unsigned int *)(s.ctr + kCtrLds) + cls;
+        HIP_CHECK(launch_lds_batches(lg, pp.dev->cus, streams[cls % nst], c.kg, c.kp, R.recs, R.batch_first,
+                                     R.batch_count, b, e, cur, ewords, c.out, nullptr));
         b = e;
         cls++;
     }
 }
 
-static void fill_result(const DevPrep &d, const beep_opts &o, const unsigned long long *acc, beep_result &res)
+void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane, unsigned long long *host_ctr)
+{
+    hipStream_t st = lane.main;
+    HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, st));
+    HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, st));
+    HIP_CHECK(hipEventRecord(lane.ev[0], st));
+    // The LDS lane classes run on the side streams, concurrently with the hub
+    // rounds on the main stream.
+    for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipStreamWaitEvent(lane.side[i], lane.ev[0], 0));
+    enqueue_lds_classes(pp, c, s, lane.side, lane.nside);
+    for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipEventRecord(lane.side_done[i], lane.side[i]));
+    enqueue_hubs(pp, c, s, st);
+    HIP_CHECK(hipEventRecord(lane.ev[1], st));
+    for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipStreamWaitEvent(st, lane.side_done[i], 0));
+    HIP_CHECK(hipEventRecord(lane.ev[2], st));
+    HIP_CHECK(hipMemcpyAsync(host_ctr, s.ctr, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipEventRecord(lane.ev[3], st));
+}
+
+void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res)
 {
     res.count = acc[kCtrAcc + 0];
     res.hash_sum = acc[kCtrAcc + 1];
@@ -392,67 +462,68 @@ static void fill_result(const DevPrep &d, const beep_opts &o, const unsigned lon
     res.n_emitted = (o.emit != BEEP_EMIT_NONE) ? acc[0] : 0;
     res.n_written = (o.emit == BEEP_EMIT_BUFFER && o.match_cap) ? std::min<uint64_t>(acc[3], o.match_cap) : 0;
     res.donations = acc[kCtrDonated];
-    (void)d;
+    res.stopped = acc[kCtrStop] ? 1u : 0u;
+}
+
+// Sizes (grow-only) and carves the owned scratch region of a single query.
+static void own_scratch(DevPrep &d, const ScratchNeeds &n)
+{
+    const uint64_t b = n.bytes();
+    if (b > d.region_bytes) {
+        if (d.region) HIP_CHECK(hipFree(d.region));
+        d.region = nullptr;
+        d.region_bytes = 0;
+        HIP_CHECK(hipMalloc((void **)&d.region, b));
+        d.region_bytes = b;
+    }
+    carve(d.scratch, d.region, n);
 }
 
 void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res)
 {
-    const int width = p.L;
-    uint32_t *dbuf = nullptr;
     upload(pp, o.device, p.skip_split);
-    HIP_CHECK(hipSetDevice(pp.dev->device));
-    if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) dbuf = dev_alloc<uint32_t>(o.match_cap * (uint64_t)width);
-    LaunchCfg c = prepare_launch(pp, p, o, dbuf);
     DevPrep &d = *pp.dev;
-    hipStream_t st = d.stream;
-    HIP_CHECK(launch_fill_zero(d.ctr, sizeof(unsigned long long) * kCtrWords, st));
-    HIP_CHECK(hipEventRecord(d.ev[0], st));
-    // The LDS lane classes run on the side streams, concurrently with the hub
-    // rounds on the main stream.
-    for (auto &s : d.side) HIP_CHECK(hipStreamWaitEvent(s, d.ev[0], 0));
-    enqueue_lds_classes(pp, c, d.side, DevPrep::kSide);
-    for (int i = 0; i < DevPrep::kSide; i++) HIP_CHECK(hipEventRecord(d.side_done[i], d.side[i]));
-    enqueue_hubs(pp, c, st);
-    HIP_CHECK(hipEventRecord(d.ev[1], st));
-    for (int i = 0; i < DevPrep::kSide; i++) HIP_CHECK(hipStreamWaitEvent(st, d.side_done[i], 0));
-    HIP_CHECK(hipEventRecord(d.ev[2], st));
+    HIP_CHECK(hipSetDevice(d.device));
+    own_scratch(d, scratch_needs(pp, p, o, d.cus));
+    d.lane.create(d.device, Lane::kMaxSide);
+    const LaunchCfg c = make_launch(pp, p, o, d.scratch, d.cus);
     unsigned long long acc[kCtrWords];
-    HIP_CHECK(hipMemcpyAsync(acc, d.ctr, sizeof(acc), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipEventRecord(d.ev[3], st));
-    HIP_CHECK(hipStreamSynchronize(st));
+    enqueue_query(pp, c, d.scratch, d.lane, acc);
+    HIP_CHECK(hipStreamSynchronize(d.lane.main));
     float t_all = 0, t_hub = 0, t_lds = 0;
-    HIP_CHECK(hipEventElapsedTime(&t_all, d.ev[0], d.ev[3]));
-    HIP_CHECK(hipEventElapsedTime(&t_hub, d.ev[0], d.ev[1]));
-    for (int i = 0; i < DevPrep::kSide; i++) {
+    HIP_CHECK(hipEventElapsedTime(&t_all, d.lane.ev[0], d.lane.ev[3]));
+    HIP_CHECK(hipEventElapsedTime(&t_hub, d.lane.ev[0], d.lane.ev[1]));
+    for (int i = 0; i < d.lane.nside; i++) {
         float t = 0;
-        HIP_CHECK(hipEventElapsedTime(&t, d.ev[0], d.side_done[i]));
+        HIP_CHECK(hipEventElapsedTime(&t, d.lane.ev[0], d.lane.side_done[i]));
         t_lds = std::max(t_lds, t);
     }
-    fill_result(d, o, acc, res);
+    fill_result(o, acc, res);
     res.t_enum_ms = t_all;
     res.t_hub_ms = t_hub;
     res.t_lds_ms = t_lds;
-    if (dbuf) {
-        if (o.match_buf && res.n_written)
-            HIP_CHECK(hipMemcpy(o.match_buf, dbuf, res.n_written * width * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        HIP_CHECK(hipFree(dbuf));
-    }
+    if (c.out.buf && o.match_buf && res.n_written)
+        HIP_CHECK(hipMemcpy(o.match_buf, c.out.buf, res.n_written * (uint64_t)p.L * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost));
 }
 
 void upload_prep(Prep &pp, int device, int skip_split) { upload(pp, device, skip_split); }
 
 // ------------------------------------------------------------------ batch
 //
-// Several prepared queries in one hipGraph: per query a counter clear and its
-// hub rounds on a branch of their own, and one persistent lds_multi_kernel
-// over every query's LDS batches, heaviest first.
+// Several prepared queries in one hipGraph: per query a counter clear, its
+// deadline and its hub rounds on a branch of their own, and one persistent
+// lds_multi_kernel over every query's LDS batches, heaviest first.
 
 struct Batch {
     int device = -1;
     std::vector<Prep *> preps;
+    std::vector<const DevPrep *> devs;  // device state the graph was built from
     std::vector<const Plan *> plans;
     std::vector<beep_opts> opts;
     std::vector<LaunchCfg> cfgs;
+    std::vector<char *> regions;        // per query scratch, owned by the batch
+    std::vector<Scratch> scr;
     MultiQuery *dqs = nullptr;
     MultiWork *dwork = nullptr;
     uint32_t nwork = 0;
@@ -470,6 +541,8 @@ struct Batch {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : {(void *)dqs, (void *)dwork, (void *)cursor})
+            if (p) (void)hipFree(p);
+        for (char *p : regions)
             if (p) (void)hipFree(p);
         for (auto e : done)
             if (e) (void)hipEventDestroy(e);
@@ -495,14 +568,22 @@ Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opt
         for (int i = 0; i < n; i++) {
             if (opts[i].emit == BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "batches do not return tuple buffers");
             if (opts[i].device != b->device) throw Error(BEEP_EINVAL, "a batch runs on one device");
+            Prep &pp = *preps[i];
+            upload(pp, opts[i].device, plans[i]->skip_split);
+            DevPrep &d = *pp.dev;
+            const ScratchNeeds need = scratch_needs(pp, *plans[i], opts[i], d.cus);
+            char *region = nullptr;
+            HIP_CHECK(hipMalloc((void **)&region, need.bytes()));
+            b->regions.push_back(region);
+            b->scr.emplace_back();
+            carve(b->scr.back(), region, need);
             b->preps.push_back(preps[i]);
+            b->devs.push_back(pp.dev);
             b->plans.push_back(plans[i]);
             b->opts.push_back(opts[i]);
-            b->cfgs.push_back(prepare_launch(*preps[i], *plans[i], opts[i], nullptr));
+            b->cfgs.push_back(make_launch(pp, *plans[i], opts[i], b->scr.back(), d.cus));
             const LaunchCfg &c = b->cfgs.back();
-            Prep &pp = *preps[i];
-            DevPrep &d = *pp.dev;
-            qs[i] = MultiQuery{c.kg, c.kp, d.recs, d.batch_first, d.batch_count, c.out};
+            qs[i] = MultiQuery{c.kg, c.kp, d.roots.recs, d.roots.batch_first, d.roots.batch_count, c.out};
             b->nlev = std::max<uint32_t>(b->nlev, (uint32_t)c.nlev);
             for (uint32_t bi = 0; bi < (uint32_t)pp.batches.size(); bi++) {
                 const LdsBatch &lb = pp.batches[bi];
@@ -542,13 +623,16 @@ static void batch_capture(Batch &b)
 {
     const int n = (int)b.preps.size();
     HIP_CHECK(hipStreamBeginCapture(b.s0, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n; i++)
-        HIP_CHECK(launch_fill_zero(b.preps[i]->dev->ctr, sizeof(unsigned long long) * kCtrWords, b.s0));
+    for (int i = 0; i < n; i++) {
+        const Scratch &s = b.scr[i];
+        HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, b.s0));
+        HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, b.cfgs[i].deadline_ticks, b.s0));
+    }
     HIP_CHECK(launch_fill_zero(b.cursor, 4, b.s0));
     HIP_CHECK(hipEventRecord(b.fork, b.s0));
     for (int i = 0; i < n; i++) {
         HIP_CHECK(hipStreamWaitEvent(b.branch[i], b.fork, 0));
-        enqueue_hubs(*b.preps[i], b.cfgs[i], b.branch[i]);
+        enqueue_hubs(*b.preps[i], b.cfgs[i], b.scr[i], b.branch[i]);
         HIP_CHECK(hipEventRecord(b.done[i], b.branch[i]));
     }
     HIP_CHECK(hipStreamWaitEvent(b.branch[n], b.fork, 0));
@@ -564,6 +648,9 @@ static void batch_capture(Batch &b)
 void batch_run(Batch &b, beep_result *res, double *t_ms)
 {
     HIP_CHECK(hipSetDevice(b.device));
+    for (size_t i = 0; i < b.preps.size(); i++)
+        if (b.preps[i]->dev != b.devs[i])
+            throw Error(BEEP_EINVAL, "a prepared query was uploaded again after the batch was created");
     if (!b.exec) batch_capture(b);
     HIP_CHECK(hipEventRecord(b.t0, b.s0));
     HIP_CHECK(hipGraphLaunch(b.exec, b.s0));
@@ -571,7 +658,7 @@ void batch_run(Batch &b, beep_result *res, double *t_ms)
     const int n = (int)b.preps.size();
     std::vector<unsigned long long> acc((size_t)n * kCtrWords);
     for (int i = 0; i < n; i++)
-        HIP_CHECK(hipMemcpyAsync(acc.data() + (size_t)i * kCtrWords, b.preps[i]->dev->ctr,
+        HIP_CHECK(hipMemcpyAsync(acc.data() + (size_t)i * kCtrWords, b.scr[i].ctr,
                                  sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, b.s0));
     HIP_CHECK(hipStreamSynchronize(b.s0));
     float t = 0;
@@ -579,7 +666,7 @@ void batch_run(Batch &b, beep_result *res, double *t_ms)
     for (int i = 0; i < n; i++) {
         if (res) {
             memset(&res[i], 0, sizeof(beep_result));
-            fill_result(*b.preps[i]->dev, b.opts[i], acc.data() + (size_t)i * kCtrWords, res[i]);
+            fill_result(b.opts[i], acc.data() + (size_t)i * kCtrWords, res[i]);
             res[i].t_enum_ms = t;
         }
     }

@@ -106,6 +106,14 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+// Deadline check: wall_clock64() is a constant-rate device clock.
+__device__ __forceinline__ bool past(uint64_t dl) { return (uint64_t)wall_clock64() > dl; }
+
+__device__ __forceinline__ void mark_stop(const KOut &out) { atomicOr(out.stop, 1ull); }
+
+// DFS steps between two deadline checks.
+constexpr uint32_t kClockEvery = 1024;
+
 // Flushes per-lane accumulators: one atomic per wave and counter.
 __device__ __forceinline__ void flush_acc(const KOut &out, uint64_t cnt, uint64_t hs, uint64_t hx)
 {
@@ -241,7 +249,8 @@ __device__ __forceinline__ LdsView lds_view(unsigned char *smem, const LdsLayout
 template <int PC>
 __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v,
                               const RootRec *__restrict__ recs, uint32_t first, uint32_t nr,
-                              const KOut &out, int Prt, uint64_t &cnt, uint64_t &hs, uint64_t &hx)
+                              const KOut &out, uint64_t dl, int Prt, uint64_t &cnt, uint64_t &hs,
+                              uint64_t &hx)
 {
     const int P = PC > 0 ? PC : Prt;
     const int tid = threadIdx.x;
@@ -333,9 +342,17 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
     }
 
     // DFS over (root, x_1) items
+    uint32_t spins = 0;
     for (;;) {
         uint32_t item = 0;
-        if (lx == 0) item = atomicAdd(&sMisc[M_NEXT], 1u);
+        if (lx == 0) {
+            if (past(dl)) {
+                item = 0xffffffffu;
+                mark_stop(out);
+            } else {
+                item = atomicAdd(&sMisc[M_NEXT], 1u);
+            }
+        }
         item = __shfl(item, 0, P);
         if (item >= sumd) break;
         const RootRec rr = sRoot[sJRoot[item]];
@@ -377,6 +394,10 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
         int l = 3;
         uint32_t pos = 0;
         for (;;) {
+            if (++spins == kClockEvery) {
+                spins = 0;
+                if (past(dl)) break;  // the next claim records the stop
+            }
             uint64_t vv = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
             const uint32_t pw = pos >> 6;
             if (w < pw) vv = 0ull;
@@ -425,12 +446,20 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
         v.sMisc[M_SM + tid] = pl.smask[tid];
     }
     uint64_t cnt = 0, hs = 0, hx = 0;
+    const uint64_t dl = *out.dl;
     for (;;) {
-        if (tid == 0) v.sMisc[M_BATCH] = b0 + atomicAdd(cursor, 1u);
+        if (tid == 0) {
+            if (past(dl)) {
+                v.sMisc[M_BATCH] = b1;
+                mark_stop(out);
+            } else {
+                v.sMisc[M_BATCH] = b0 + atomicAdd(cursor, 1u);
+            }
+        }
         __syncthreads();
         const uint32_t b = v.sMisc[M_BATCH];
         if (b >= b1) break;
-        process_batch<P>(g, pl, v, recs, batch_first[b], batch_count[b], out, P, cnt, hs, hx);
+        process_batch<P>(g, pl, v, recs, batch_first[b], batch_count[b], out, dl, P, cnt, hs, hx);
     }
     flush_acc(out, cnt, hs, hx);
 }
@@ -461,7 +490,8 @@ __global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__r
         }
         __syncthreads();
         uint64_t cnt = 0, hs = 0, hx = 0;
-        process_batch<0>(q.g, pl, v, q.recs, q.batch_first[w.batch], q.batch_count[w.batch], q.out,
+        const uint64_t dl = *q.out.dl;
+        process_batch<0>(q.g, pl, v, q.recs, q.batch_first[w.batch], q.batch_count[w.batch], q.out, dl,
                          1 << w.lanes_log2, cnt, hs, hx);
         flush_acc(q.out, cnt, hs, hx);
     }
@@ -608,6 +638,8 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     else my = stk + ((uint64_t)blockIdx.x * (kBlock / 64) + wv) * (uint64_t)(nlev + 1) * wmax;
     uint64_t *e1 = my + (uint64_t)nlev * wmax;
     uint64_t cnt = 0, hs = 0, hx = 0;
+    const uint64_t dl = *out.dl;
+    uint32_t spins = 0;
     auto xget = [&](int q) -> uint32_t { return sX[wv][q]; };
 
     // per-item context
@@ -651,6 +683,10 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         int l = 3;
         uint32_t pos = pos0;
         for (;;) {
+            if (++spins == kClockEvery) {
+                spins = 0;
+                if (past(dl)) break;  // the next claim records the stop
+            }
             uint32_t xi = 0xffffffffu;
             uint32_t pw = pos >> 6;
             const uint32_t wl = sWlo[wv][l - 1];
@@ -725,7 +761,14 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         const uint64_t nq = ntail < hq.cap ? ntail : hq.cap;
         for (;;) {
             unsigned long long c = 0;
-            if (lane == 0) c = atomicAdd(hq.head, 1ull);
+            if (lane == 0) {
+                if (past(dl)) {
+                    c = ~0ull;
+                    mark_stop(out);
+                } else {
+                    c = atomicAdd(hq.head, 1ull);
+                }
+            }
             c = __shfl(c, 0, 64);
             if (c >= nq) break;
             const HubChunk ch = hq.q[c];
@@ -746,8 +789,16 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     for (;;) {
         if (r >= claim_end) {
             unsigned long long it = 0;
-            if (lane == 0) it = atomicAdd(cursor, (unsigned long long)kClaim);
+            if (lane == 0) {
+                if (past(dl)) {
+                    it = ~0ull;
+                    mark_stop(out);
+                } else {
+                    it = atomicAdd(cursor, (unsigned long long)kClaim);
+                }
+            }
             it = __shfl(it, 0, 64);
+            if (it == ~0ull) break;
             r = i0 + it;
             claim_end = r + kClaim < i1 ? r + kClaim : i1;
         }
@@ -902,6 +953,18 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st)
     uint64_t blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(fill_zero_u32_kernel, dim3((uint32_t)blocks), dim3(kBlock), 0, st, (uint32_t *)p, n);
+    return hipGetLastError();
+}
+
+// Writes the deadline of one launch sequence: now + ticks, or none.
+__global__ void arm_deadline_kernel(unsigned long long *dl, uint64_t ticks)
+{
+    if (threadIdx.x == 0) *dl = ticks ? (unsigned long long)wall_clock64() + ticks : ~0ull;
+}
+
+hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks, hipStream_t st)
+{
+    hipLaunchKernelGGL(arm_deadline_kernel, dim3(1), dim3(64), 0, st, dl, ticks);
     return hipGetLastError();
 }
 

@@ -87,6 +87,11 @@ struct KOut {
     unsigned long long *acc;  // [0] count [1] hash_sum [2] hash_xor [3] written
     uint32_t *buf;            // device tuple buffer (cap * L words) or null
     uint64_t cap;
+    // Deadline in wall_clock64() ticks, written on the device by the launch
+    // sequence's first kernel (all ones: none).  Kernels stop claiming work and
+    // leave their DFS once it has passed, and set *stop.
+    const unsigned long long *dl;
+    unsigned long long *stop;
 };
 
 // One prepared query of a multi-query launch.

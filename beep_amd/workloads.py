@@ -10,7 +10,8 @@ sorted, i.e. what the reference reads from a (src, dst)-sorted .bel file.
                -- 4-clique then diamond, sgm, one query at a time
   rmat22cut    R-MAT scale 22 with ids >= 3,774,768 rejected, (.57,.19,.19,.05),
                seed 3 -- the 8 named central-node patterns, batched
-  chunglu3m    Chung-Lu n = 3,072,441, gamma 2.0, seed 4 -- serving stream
+  chunglu3m    Chung-Lu n = 3,072,441, gamma 2.0, i0 700, 121,350,000 samples, seed 4
+               (about 117M undirected edges, max degree about 34k) -- serving stream
   rmat26       R-MAT scale 26, 27 * 2^26 samples, seed 5 -- large single GPU
 """
 from . import Graph
@@ -24,7 +25,7 @@ WORKLOADS = {
                       queries=[("triangle", "sgc"), ("tailed_triangle", "sgc"), ("diamond", "sgc"),
                                ("4clique", "sgc"), ("4star", "sgc"), ("w5", "sgc"),
                                ("5clique", "sgc"), ("6clique", "sgc")]),
-    "chunglu3m": dict(kind="chung_lu", n=3072441, samples=121_000_000, gamma=2.0, i0=8.0, seed=4,
+    "chunglu3m": dict(kind="chung_lu", n=3072441, samples=121_350_000, gamma=2.0, i0=700.0, seed=4,
                       queries=[]),
     "rmat26": dict(kind="rmat", scale=26, samples=27 << 26, seed=5, n_keep=0,
                    queries=[("4clique", "sgc"), ("w5", "sgc"), ("5clique", "sgc")]),

@@ -16,7 +16,7 @@
  * Conventions: opaque handles with caller-owned *_free; every entry point
  * returns 0 on success and a negative errno-style code otherwise (never exits);
  * beep_last_error() describes the last failure of the calling thread.  Handles
- * are not thread-safe except a beep_server, whose submit/poll are MPSC-safe.
+ * are not thread-safe except a beep_server, whose submit/poll/stats are.
  * Device memory belongs to the handle that allocated it.
  */
 #ifndef BEEP_H
@@ -161,6 +161,8 @@ typedef struct {
     int donate;             /* 0 = off; 1 = donate level-2 subtrees of hub roots */
     uint32_t donate_min;    /* donation threshold: min |C_2| of a donated subtree */
     int timing;             /* 1 = record HIP events around the enumeration */
+    double deadline_ms;     /* > 0: stop the enumeration this long after it starts on
+                               the device; the result is then partial and marked stopped */
 } beep_opts;
 
 typedef struct {
@@ -170,9 +172,11 @@ typedef struct {
     double t_enum_ms;       /* first enumeration launch -> counter readback */
     double t_hub_ms, t_lds_ms; /* device time of the hub and LDS phases */
     uint64_t donations;     /* subtrees donated through the ring */
+    uint64_t stopped;       /* 1: the deadline passed; counts are partial */
 } beep_result;
 
 void beep_opts_default(beep_opts *o);
+/* Returns BEEP_ETIMEDOUT (with the partial result filled in) when the deadline passed. */
 int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_result *out);
 
 /* ------------------------------------------------------------------------ */
@@ -192,6 +196,57 @@ int beep_batch_create(beep_prep *const *preps, const beep_plan *const *plans, co
 /* results: n entries (t_enum_ms is the time of the whole graph launch). */
 int beep_batch_run(beep_batch *b, beep_result *results, double *t_ms);
 void beep_batch_free(beep_batch *b);
+
+/* ------------------------------------------------------------------------ */
+/* Serving.  No counterpart in the reference (it reloads and re-prepares the */
+/* graph for every query, main.cu:70-341).  A server keeps one data graph    */
+/* resident: peeled/oriented variants and prepared root queues are cached in */
+/* HBM and shared by every query that needs them.  Submitted queries wait in */
+/* arrival order; the oldest is admitted as soon as a lane (two streams) is  */
+/* free and its predicted scratch footprint -- hub bitmap arena, hash        */
+/* tables, DFS stacks, donation ring, tuple buffer -- fits the free part of  */
+/* the server's HBM pool.  Admitted queries run concurrently.  submit, poll  */
+/* and stats may be called from any thread.                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int device;
+    int lanes;              /* queries running at once, 1..16 */
+    uint64_t pool_bytes;    /* scratch pool; 0 = a quarter of the free HBM */
+    uint64_t arena_cap;     /* hub-arena bytes per query; 0 = pool / (2 * lanes) */
+} beep_server_opts;
+
+typedef struct {
+    uint64_t ticket;
+    int status;             /* 0, BEEP_ETIMEDOUT (deadline passed, partial counts) or an error */
+    beep_result result;     /* t_enum_ms: device time of the query's launch sequence */
+    double queue_ms;        /* submit -> admission */
+    double service_ms;      /* admission -> completion */
+    double latency_ms;      /* submit -> completion */
+    uint64_t footprint;     /* predicted scratch bytes the query was admitted with */
+} beep_server_done;
+
+typedef struct {
+    uint64_t submitted, completed, stopped, failed;
+    uint64_t pending, running, peak_running;
+    uint64_t pool_bytes, pool_used, pool_peak;
+    uint64_t variants, root_queues;
+} beep_server_stats;
+
+void beep_server_opts_default(beep_server_opts *o);
+/* g must outlive the server. */
+int beep_server_create(const beep_graph *g, const beep_server_opts *o, beep_server **out);
+/* Builds (if needed) and uploads the variant and root queue p needs. */
+int beep_server_prepare(beep_server *s, const beep_plan *p);
+/* o: emit, match_cap, match_buf (kept valid by the caller until the ticket is
+ * polled), lanes, donate, donate_min, deadline_ms; o->device is ignored. */
+int beep_server_submit(beep_server *s, const beep_plan *p, const beep_opts *o, uint64_t *ticket);
+/* Takes one finished query; BEEP_ETIMEDOUT when none finished within
+ * timeout_ms (< 0: wait without limit). */
+int beep_server_poll(beep_server *s, beep_server_done *out, int timeout_ms);
+int beep_server_stats_get(beep_server *s, beep_server_stats *out);
+/* Waits for every submitted query, then releases the server's memory. */
+void beep_server_free(beep_server *s);
 
 /* Tuple hash used by BEEP_EMIT_HASH: sum and xor over emitted tuples. */
 uint64_t beep_tuple_hash(const uint32_t *t, int len);
