@@ -17,9 +17,10 @@ SGM, SGC = 0, 1
 EMIT_NONE, EMIT_HASH, EMIT_BUFFER = 0, 1, 2
 
 __all__ = ["Graph", "Plan", "Prepared", "Batch", "Server", "SG_Match", "SGM", "SGC", "BeepError",
-           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "ETIMEDOUT", "tuple_hash", "mode_of"]
+           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "ETIMEDOUT", "ERANGE", "tuple_hash", "mode_of"]
 
 ETIMEDOUT = -110
+ERANGE = -34
 
 
 def mode_of(task):
@@ -257,10 +258,11 @@ class Server:
     service and end-to-end latencies in ms.
     """
 
-    def __init__(self, graph, device=0, lanes=4, pool_bytes=0, arena_cap=0):
+    def __init__(self, graph, device=0, lanes=4, pool_bytes=0, arena_cap=0, budget_ms=0.0, pilot_stride=0):
         o = _ffi.ServerOpts()
         lib().beep_server_opts_default(C.byref(o))
         o.device, o.lanes, o.pool_bytes, o.arena_cap = device, lanes, pool_bytes, arena_cap
+        o.budget_ms, o.pilot_stride = float(budget_ms), pilot_stride
         h = C.c_void_p()
         check(lib().beep_server_create(graph._h, C.byref(o), C.byref(h)))
         self._h = h
@@ -293,7 +295,7 @@ class Server:
         plan, _, buf = self._keep.pop(d.ticket, (None, None, None))
         out = _result_dict(d.result)
         out.update(ticket=d.ticket, status=d.status, queue_ms=d.queue_ms, service_ms=d.service_ms,
-                   latency_ms=d.latency_ms, footprint=d.footprint)
+                   latency_ms=d.latency_ms, footprint=d.footprint, predicted_ms=d.predicted_ms)
         if buf is not None and plan is not None:
             out["matches"] = buf[:d.result.n_written * plan.width].reshape(-1, plan.width)
         return out

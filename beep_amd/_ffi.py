@@ -48,19 +48,20 @@ class Result(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("lanes", C.c_int), ("pool_bytes", C.c_uint64),
-                ("arena_cap", C.c_uint64)]
+                ("arena_cap", C.c_uint64), ("budget_ms", C.c_double), ("pilot_stride", C.c_uint32)]
 
 
 class ServerDone(C.Structure):
     _fields_ = [("ticket", C.c_uint64), ("status", C.c_int), ("result", Result),
                 ("queue_ms", C.c_double), ("service_ms", C.c_double), ("latency_ms", C.c_double),
-                ("footprint", C.c_uint64)]
+                ("footprint", C.c_uint64), ("predicted_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [(f, C.c_uint64) for f in ("submitted", "completed", "stopped", "failed", "pending",
                                           "running", "peak_running", "pool_bytes", "pool_used",
-                                          "pool_peak", "variants", "root_queues")]
+                                          "pool_peak", "variants", "root_queues", "refused",
+                                          "pilots")]
 
 
 # Every function declared in include/beep.h with its ctypes signature.

@@ -169,8 +169,9 @@ struct Prep {
 // Fresh variant + root queue.
 std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p);
 // Root queue over an existing variant (which must match the plan's
-// min_qdeg and asc).
-std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &v, const Plan &p);
+// min_qdeg and asc).  sample_stride > 1 keeps every sample_stride-th entry of
+// the work list (ordered by row length, so the sample is stratified by size).
+std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &v, const Plan &p, uint32_t sample_stride = 1);
 
 // LDS budget of the batch kernel (words / slots); shared with the kernels.
 struct LdsBudget {

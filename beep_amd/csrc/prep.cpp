@@ -164,7 +164,7 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p)
     return prepare_on(make_variant(g, p.min_qdeg, p.asc), p);
 }
 
-std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan &p)
+std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan &p, uint32_t sample_stride)
 {
     if (!vp || vp->min_qdeg != p.min_qdeg || vp->asc != p.asc)
         throw Error(BEEP_EINVAL, "variant does not match the plan");
@@ -229,6 +229,11 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
         // need a (stable) sort
         if (pp->reduced)
             std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return len_of(a) > len_of(b); });
+        if (sample_stride > 1) {
+            size_t o = 0;
+            for (size_t i = 0; i < idx.size(); i += sample_stride) idx[o++] = idx[i];
+            idx.resize(o);
+        }
         pp->wu.resize(idx.size());
         pp->woff.resize(idx.size());
         pp->wlen.resize(idx.size());

@@ -12,11 +12,16 @@
 // footprint fits the free part of the pool, so queries that arrive while
 // others run are started right away and share the GPU with them.  A query
 // can carry a deadline after which its kernels stop (the result is then
-// marked stopped).
+// marked stopped).  With a budget, each new plan first runs as a pilot on a
+// stratified 1/stride sample of its root queue (deadline budget/stride, on
+// the lane the query was admitted to); the pilot's device time times the
+// stride predicts the full run, and plans predicted above the budget are
+// refused without running.  Predictions are cached per plan.
 
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -91,6 +96,10 @@ struct Server {
         beep_opts opts{};
         Clock::time_point t_submit, t_admit;
         std::shared_ptr<Prep> pp;
+        std::shared_ptr<Prep> pilot;  // sampled root queue while a pilot runs first
+        bool piloting = false;
+        double predicted_ms = -1.0;
+        std::string sig;
         ScratchNeeds need;
         uint64_t off = 0, bytes = 0;
     };
@@ -107,6 +116,8 @@ struct Server {
     const Graph *g = nullptr;
     int device = 0, cus = 0;
     uint64_t pool_bytes = 0, arena_cap = 0;
+    double budget_ms = 0;
+    uint32_t stride = 32;
     char *pool = nullptr;
     PoolAlloc alloc;
     std::vector<std::unique_ptr<Slot>> slots;
@@ -119,33 +130,88 @@ struct Server {
     uint64_t next_ticket = 1;
     beep_server_stats st{};
 
-    std::mutex cache_mu;  // variants, rootq
+    std::mutex cache_mu;  // variants, rootq, pilotq
     std::map<std::pair<int, int>, std::shared_ptr<Variant>> variants;
-    std::map<RootKey, std::shared_ptr<Prep>> rootq;
+    std::map<RootKey, std::shared_ptr<Prep>> rootq, pilotq;
 
     std::deque<Job> pending;  // worker only
+    std::map<std::string, double> predicted;  // worker only: plan -> predicted device ms
     std::thread worker;
 
     ~Server();
-    std::shared_ptr<Prep> get_prep(const Plan &p);
+    std::shared_ptr<Prep> get_prep(const Plan &p, uint32_t sample_stride = 1);
     void run();
+    void launch(Slot &sl);
+    beep_opts pilot_opts(const beep_opts &o) const;
     void finish(Slot &sl, int status, const std::string &err);
     void fail(Job &j, int status);
 };
 
-std::shared_ptr<Prep> Server::get_prep(const Plan &p)
+std::shared_ptr<Prep> Server::get_prep(const Plan &p, uint32_t sample_stride)
 {
     std::lock_guard<std::mutex> lk(cache_mu);
     const RootKey key{p.min_qdeg, p.asc, p.max_qdeg, p.above_split, p.rows_above_split, p.uses_bitmap,
                       p.skip_split};
-    auto it = rootq.find(key);
-    if (it != rootq.end()) return it->second;
+    auto &cache = sample_stride > 1 ? pilotq : rootq;
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
     auto &v = variants[{p.min_qdeg, p.asc}];
     if (!v) v = make_variant(*g, p.min_qdeg, p.asc);
-    std::shared_ptr<Prep> pp(prepare_on(v, p).release());
+    std::shared_ptr<Prep> pp(prepare_on(v, p, sample_stride).release());
     upload(*pp, device, p.skip_split, arena_cap);
-    rootq[key] = pp;
+    cache[key] = pp;
     return pp;
+}
+
+// Everything of a plan the enumeration depends on.
+static std::string plan_signature(const Plan &p)
+{
+    std::string s;
+    auto put = [&](const void *b, size_t n) { s.append((const char *)b, n); };
+    const int head[] = {p.k, p.mode, p.lunmat, p.asc, p.max_qdeg, p.min_qdeg, p.L};
+    put(head, sizeof(head));
+    put(p.seq, sizeof(p.seq));
+    put(p.qedge_ptr, sizeof(p.qedge_ptr));
+    put(p.qedge, sizeof(p.qedge));
+    put(p.sym_ptr, sizeof(p.sym_ptr));
+    put(p.sym, sizeof(p.sym));
+    return s;
+}
+
+static ScratchNeeds max_needs(const ScratchNeeds &a, const ScratchNeeds &b)
+{
+    ScratchNeeds n;
+    n.arena_words = std::max(a.arena_words, b.arena_words);
+    n.hash_slots = std::max(a.hash_slots, b.hash_slots);
+    n.stk_words = std::max(a.stk_words, b.stk_words);
+    n.queue_cap = std::max(a.queue_cap, b.queue_cap);
+    n.slot_cap = std::max(a.slot_cap, b.slot_cap);
+    n.dbuf_words = std::max(a.dbuf_words, b.dbuf_words);
+    return n;
+}
+
+beep_opts Server::pilot_opts(const beep_opts &o) const
+{
+    beep_opts po = o;
+    po.emit = BEEP_EMIT_NONE;
+    po.match_cap = 0;
+    po.match_buf = nullptr;
+    po.deadline_ms = budget_ms / stride;
+    return po;
+}
+
+// Enqueues the slot's current stage: the pilot, or the query itself.
+void Server::launch(Slot &sl)
+{
+    Job &j = sl.job;
+    if (j.piloting) {
+        const beep_opts po = pilot_opts(j.opts);
+        sl.c = make_launch(*j.pilot, j.plan, po, sl.s, cus);
+        enqueue_query(*j.pilot, sl.c, sl.s, sl.lane, sl.host_ctr);
+    } else {
+        sl.c = make_launch(*j.pp, j.plan, j.opts, sl.s, cus);
+        enqueue_query(*j.pp, sl.c, sl.s, sl.lane, sl.host_ctr);
+    }
 }
 
 void Server::fail(Job &j, int status)
@@ -154,10 +220,12 @@ void Server::fail(Job &j, int status)
     d.ticket = j.ticket;
     d.status = status;
     d.footprint = j.bytes;
+    d.predicted_ms = j.predicted_ms;
     const auto now = Clock::now();
     d.latency_ms = ms_between(j.t_submit, now);
     std::lock_guard<std::mutex> lk(mu);
-    st.failed++;
+    if (status == BEEP_ERANGE) st.refused++;
+    else st.failed++;
     st.pending--;
     done.push_back(d);
     cv_out.notify_all();
@@ -169,6 +237,7 @@ void Server::finish(Slot &sl, int status, const std::string &err)
     beep_server_done d{};
     d.ticket = j.ticket;
     d.footprint = j.bytes;
+    d.predicted_ms = j.predicted_ms;
     if (status == 0) {
         try {
             fill_result(j.opts, sl.host_ctr, d.result);
@@ -202,11 +271,13 @@ void Server::finish(Slot &sl, int status, const std::string &err)
     alloc.release(j.off, j.bytes);
     sl.busy = false;
     j.pp.reset();
+    j.pilot.reset();
     std::lock_guard<std::mutex> lk(mu);
     st.running--;
     st.pool_used = alloc.used();
     if (status == 0) st.completed++;
     else if (status == BEEP_ETIMEDOUT) st.stopped++;
+    else if (status == BEEP_ERANGE) st.refused++;
     else st.failed++;
     done.push_back(d);
     cv_out.notify_all();
@@ -234,8 +305,29 @@ void Server::run()
             if (!sl.busy) continue;
             const hipError_t q = hipEventQuery(sl.lane.ev[3]);
             if (q == hipErrorNotReady) continue;
-            finish(sl, q == hipSuccess ? 0 : BEEP_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
             progress = true;
+            if (q == hipSuccess && sl.job.piloting) {
+                // pilot done: predict, then refuse or run the query on the same lane
+                float t = 0;
+                const bool ok = hipEventElapsedTime(&t, sl.lane.ev[0], sl.lane.ev[3]) == hipSuccess;
+                const bool stopped = sl.host_ctr[kCtrStop] != 0;
+                const double pred = (stopped || !ok) ? INFINITY : (double)t * stride;
+                predicted[sl.job.sig] = pred;
+                sl.job.predicted_ms = pred;
+                sl.job.piloting = false;
+                if (pred > budget_ms) {
+                    finish(sl, BEEP_ERANGE, "predicted device time above the budget");
+                    continue;
+                }
+                try {
+                    launch(sl);
+                } catch (const Error &e) {
+                    (void)hipStreamSynchronize(sl.lane.main);
+                    finish(sl, e.code, e.what());
+                }
+                continue;
+            }
+            finish(sl, q == hipSuccess ? 0 : BEEP_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
         }
         // admissions, oldest first
         while (!pending.empty()) {
@@ -244,6 +336,19 @@ void Server::run()
                 try {
                     j.pp = get_prep(j.plan);
                     j.need = scratch_needs(*j.pp, j.plan, j.opts, cus);
+                    if (budget_ms > 0) {
+                        j.sig = plan_signature(j.plan);
+                        auto it = predicted.find(j.sig);
+                        if (it != predicted.end()) {
+                            j.predicted_ms = it->second;
+                            if (j.predicted_ms > budget_ms)
+                                throw Error(BEEP_ERANGE, "predicted device time above the budget");
+                        } else {
+                            j.pilot = get_prep(j.plan, stride);
+                            j.need = max_needs(j.need, scratch_needs(*j.pilot, j.plan, pilot_opts(j.opts), cus));
+                            j.piloting = true;
+                        }
+                    }
                     j.bytes = j.need.bytes();
                     if (j.bytes > pool_bytes)
                         throw Error(BEEP_ENOMEM, "the query's predicted scratch exceeds the server pool");
@@ -280,8 +385,11 @@ void Server::run()
             }
             try {
                 carve(sl.s, pool + off, sl.job.need);
-                sl.c = make_launch(*sl.job.pp, sl.job.plan, sl.job.opts, sl.s, cus);
-                enqueue_query(*sl.job.pp, sl.c, sl.s, sl.lane, sl.host_ctr);
+                if (sl.job.piloting) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    st.pilots++;
+                }
+                launch(sl);
             } catch (const Error &e) {
                 (void)hipStreamSynchronize(sl.lane.main);
                 finish(sl, e.code, e.what());
@@ -329,6 +437,9 @@ Server *server_create(const Graph *g, const beep_server_opts &o)
     s->pool_bytes &= ~255ull;
     if (s->pool_bytes < (1ull << 20)) throw Error(BEEP_EINVAL, "pool too small");
     s->arena_cap = o.arena_cap ? o.arena_cap : s->pool_bytes / (2ull * o.lanes);
+    if (!(o.budget_ms >= 0)) throw Error(BEEP_EINVAL, "bad budget");
+    s->budget_ms = o.budget_ms;
+    s->stride = o.pilot_stride ? o.pilot_stride : 32;
     HIP_CHECK(hipMalloc((void **)&s->pool, s->pool_bytes));
     s->alloc.reset(s->pool_bytes);
     s->st.pool_bytes = s->pool_bytes;
@@ -343,7 +454,11 @@ Server *server_create(const Graph *g, const beep_server_opts &o)
     return s.release();
 }
 
-void server_prepare(Server &s, const Plan &p) { (void)s.get_prep(p); }
+void server_prepare(Server &s, const Plan &p)
+{
+    (void)s.get_prep(p);
+    if (s.budget_ms > 0) (void)s.get_prep(p, s.stride);
+}
 
 uint64_t server_submit(Server &s, const Plan &p, const beep_opts &o)
 {
@@ -382,7 +497,7 @@ void server_stats(Server &s, beep_server_stats &out)
     }
     std::lock_guard<std::mutex> lk(s.cache_mu);
     out.variants = s.variants.size();
-    out.root_queues = s.rootq.size();
+    out.root_queues = s.rootq.size() + s.pilotq.size();
 }
 
 void server_free(Server *s) { delete s; }

@@ -30,7 +30,7 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                              const uint32_t *hor, uint64_t g0, uint64_t g1, const uint32_t *keys,
-                             const uint32_t *vals, uint64_t *E);
+                             const uint32_t *vals, uint64_t *E, const KOut &out);
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
@@ -384,6 +384,7 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
 // Hub rounds on one stream: clear, hash, bitmap rows, DFS + donation drain.
 void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t st)
 {
+    if (c.kp.L < 3) return;  // k = 3 with an unmaterialised last level: no level to count
     const DevRoots &R = pp.dev->roots;
     const uint32_t wmax = std::max<uint32_t>(R.wmax, 1);
     const int cus = pp.dev->cus;
@@ -397,7 +398,7 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
             HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.keys, s.vals));
             int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
             HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.keys,
-                                        s.vals, s.arena));
+                                        s.vals, s.arena, c.out));
         }
         unsigned long long *cur = s.ctr + kCtrHub + (ri % 64);
         if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, 8, st));
@@ -418,6 +419,7 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
 static void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams,
                                 int nst)
 {
+    if (c.kp.L < 3) return;
     const DevRoots &R = pp.dev->roots;
     uint32_t b = 0;
     int cls = 0;
@@ -585,7 +587,7 @@ Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opt
             const LaunchCfg &c = b->cfgs.back();
             qs[i] = MultiQuery{c.kg, c.kp, d.roots.recs, d.roots.batch_first, d.roots.batch_count, c.out};
             b->nlev = std::max<uint32_t>(b->nlev, (uint32_t)c.nlev);
-            for (uint32_t bi = 0; bi < (uint32_t)pp.batches.size(); bi++) {
+            for (uint32_t bi = 0; c.kp.L >= 3 && bi < (uint32_t)pp.batches.size(); bi++) {
                 const LdsBatch &lb = pp.batches[bi];
                 uint64_t cost = 0;
                 for (uint32_t r = 0; r < lb.count; r++) {

@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                                                             uint64_t g0, uint64_t g1,
                                                             const uint32_t *__restrict__ keys,
                                                             const uint32_t *__restrict__ vals,
-                                                            uint64_t *E)
+                                                            uint64_t *E, KOut out)
 {
     __shared__ uint64_t sBeg[kBlock];
     __shared__ uint32_t sPre[kBlock];
@@ -537,7 +537,12 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t nchunks = (g1 - g0 + 63) / 64;
     const uint64_t wstride = (uint64_t)gridDim.x * (kBlock / 64);
+    const uint64_t dl = *out.dl;
     for (uint64_t c = blockIdx.x * (uint64_t)(kBlock / 64) + wv; c < nchunks; c += wstride) {
+        if (past(dl)) {  // the bitmap stays incomplete; the DFS that follows stops at once
+            if (lane == 0) mark_stop(out);
+            break;
+        }
         const uint64_t r = g0 + c * 64 + lane;
         uint32_t len = 0;
         if (r < g1) {
@@ -994,10 +999,10 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
 
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                              const uint32_t *hor, uint64_t g0, uint64_t g1, const uint32_t *keys,
-                             const uint32_t *vals, uint64_t *E)
+                             const uint32_t *vals, uint64_t *E, const KOut &out)
 {
     hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, keys,
-                       vals, E);
+                       vals, E, out);
     return hipGetLastError();
 }
 

@@ -17,6 +17,14 @@ Reported:
                OpenMP over roots) on the same prepared graphs, all host cores
                up to 16, same two queries
   parity       GPU counts and tuple-multiset hashes equal to the oracle's
+
+Other modes (each prints one JSON line of its own):
+  --mode batched  rmat22cut: the 8 named central-node patterns (sgc), one
+                  plain launch sequence per query vs one captured hipGraph with
+                  autotuned group widths / donation thresholds
+  --mode serve    chunglu3m: a Poisson stream over the 207 central-node
+                  patterns (20% sgm with a 1e6 tuple buffer) through the
+                  serving loop vs the plain path, p50/p99 latency and queries/s
 """
 import argparse
 import json
@@ -35,7 +43,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="chunglu1m")
+    ap.add_argument("--mode", choices=("plain", "batched", "serve"), default="plain")
+    ap.add_argument("--workload", default=None)
+    ap.add_argument("--queries", type=int, default=160, help="serve: queries in the stream")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve: arrival rate (0: 0.8 x plain capacity)")
+    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--deadline-ms", type=float, default=1000.0, help="serve: per-query deadline")
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--cold-sample", type=int, default=4, help="serve: queries timed the reference's way")
+    ap.add_argument("--tune-cap-ms", type=float, default=1000.0,
+                    help="batched: queries slower than this alone keep the default launch parameters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--once", action="store_true",
@@ -84,6 +101,11 @@ def main():
               "(the path is single-GPU by design; see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+    if args.mode == "batched":
+        return run_batched(args)
+    if args.mode == "serve":
+        return run_serve(args)
+    args.workload = args.workload or "chunglu1m"
     import numpy as np
 
     import beep_amd as B
@@ -206,6 +228,265 @@ def main():
                                for i, (q, _) in enumerate(queries)},
         "host_s": {"generate": round(t_gen, 3), "prepare": round(t_prep, 3),
                    "upload": round(t_upload, 3)},
+    }
+    print(json.dumps(out))
+
+
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def run_batched(args):
+    """Eight named patterns (sgc) on rmat22cut: plain launches vs one hipGraph."""
+    import beep_amd as B
+    from beep_amd import autotune as T
+    from beep_amd import patterns as P
+    from beep_amd import workloads as W
+
+    name = args.workload or "rmat22cut"
+    wl = W.WORKLOADS[name]
+    t0 = time.time()
+    g = W.make_graph(name)
+    print(f"[bench] generated {name} in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    queries = wl["queries"]
+    preps = []
+    t0 = time.time()
+    for q, task in queries:
+        s, d = P.directed(P.read_mtx_edges(q))
+        pp = B.Prepared(g, B.Plan.from_edges(s, d, B.mode_of(task)))
+        pp.upload(0)
+        preps.append(pp)
+    t_prep = time.time() - t0
+    print(f"[bench] prepared {len(preps)} queries in {t_prep:.1f} s", file=sys.stderr, flush=True)
+    # plain: one launch sequence per query, one after the other
+    plain = []
+    for (q, _), pp in zip(queries, preps):
+        pp.count()  # warm
+        r = pp.count()
+        plain.append(r)
+        print(f"[bench] plain {q}: {r['t_enum_ms']:.2f} ms count {r['count']}", file=sys.stderr, flush=True)
+    plain_ms = sum(r["t_enum_ms"] for r in plain)
+    # autotune (group width x donation threshold) on the queries that are cheap to sweep
+    tuned = []
+    t0 = time.time()
+    for (q, _), pp, r in zip(queries, preps, plain):
+        if r["t_enum_ms"] > args.tune_cap_ms:
+            tuned.append({"lanes": 0, "donate_min": 64, "ms": r["t_enum_ms"], "tuned": False})
+            continue
+        best, ms, _ = T.tune(pp, reps=1)
+        tuned.append(dict(best, ms=ms, tuned=True))
+        print(f"[bench] tuned {q}: {best} {ms:.2f} ms", file=sys.stderr, flush=True)
+    t_tune = time.time() - t0
+    batch = B.Batch(preps, tune=tuned)
+    res, t_first = batch.run()
+    ts = []
+    for _ in range(max(1, args.steps // 4)):
+        res, t = batch.run()
+        ts.append(t)
+        print(f"[bench] batched run {t:.2f} ms", file=sys.stderr, flush=True)
+    batched_ms = min(ts)
+    counts = [r["count"] for r in res]
+    parity = counts == [r["count"] for r in plain]
+    golden = os.path.join(ROOT, "tests", "golden", f"{name}_counts.json")
+    if os.path.exists(golden):
+        with open(golden) as f:
+            ref = json.load(f)
+        parity = parity and all(ref.get(q, c) == c for (q, _), c in zip(queries, counts))
+    out = {
+        "metric": "batched queries/s",
+        "value": len(queries) / (batched_ms / 1000.0),
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": len(ts), "warmup": 1,
+        "ms_per_step": batched_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"{name}: R-MAT scale 22, ids >= 3,774,768 rejected, seed 3, "
+                               f"{g.m // 2} undirected edges; 8 central-node patterns, sgc",
+                   "queries": [q for q, _ in queries], "counts": counts},
+        "plain": {"value": len(queries) / (plain_ms / 1000.0), "unit": "queries/s", "device_ms": plain_ms,
+                  "per_query_ms": {q: r["t_enum_ms"] for (q, _), r in zip(queries, plain)}},
+        "batched_ms_first_launch": t_first,
+        "speedup_vs_plain": plain_ms / batched_ms,
+        "autotune": {"params": {q: t for (q, _), t in zip(queries, tuned)}, "seconds": round(t_tune, 2),
+                     "space": {"lanes": list(T.LANES), "donate_min": list(T.DONATE_MIN)}},
+        "parity": parity,
+        "host_s": {"prepare": round(t_prep, 2)},
+    }
+    print(json.dumps(out))
+
+
+def run_serve(args):
+    """Poisson stream over the 207 central-node patterns through the server.
+
+    plain  : the same stream, one query at a time (one launch sequence each)
+             on the resident graph, every query run until done or its deadline
+    served : the serving loop -- lanes running concurrently, footprint
+             admission, and a pilot-based prediction that refuses plans
+             predicted above the deadline before they occupy a lane
+    cold   : a sample of the stream the way the reference serves a query
+             (prepare the graph for the query, upload, count, free)
+    """
+    import threading
+
+    import numpy as np
+
+    import beep_amd as B
+    from beep_amd import patterns as P
+    from beep_amd import workloads as W
+
+    name = args.workload or "chunglu3m"
+    wl = W.WORKLOADS[name]
+    t0 = time.time()
+    g = W.make_graph(name)
+    print(f"[bench] generated {name} in {time.time() - t0:.1f} s ({g.m // 2} edges)", file=sys.stderr, flush=True)
+    pats = P.central207()
+    rng = np.random.default_rng(args.seed)
+    N = args.queries
+    picks = rng.integers(0, len(pats), size=N)
+    sgm = rng.random(N) < 0.2
+    plans = {}
+    for i in range(N):
+        key = (int(picks[i]), bool(sgm[i]))
+        if key not in plans:
+            s, d = P.directed(pats[key[0]][1])
+            plans[key] = B.Plan.from_edges(s, d, B.SGM if key[1] else B.SGC)
+    stream = [(int(picks[i]), bool(sgm[i])) for i in range(N)]
+    cap = 1_000_000
+    dl = args.deadline_ms
+
+    def submit(srv, key):
+        if key[1]:
+            return srv.submit(plans[key], emit=B.EMIT_BUFFER, match_cap=cap, deadline_ms=dl)
+        return srv.submit(plans[key], deadline_ms=dl)
+
+    def drain(srv, n, tag):
+        out = {}
+        while len(out) < n:
+            r = srv.poll(timeout_ms=1000)
+            if r is not None:
+                r.pop("matches", None)
+                out[r["ticket"]] = r
+                if len(out) % 20 == 0:
+                    print(f"[bench] {tag} {len(out)}/{n}", file=sys.stderr, flush=True)
+        return out
+
+    # plain: one query at a time on the resident graph
+    plain_srv = B.Server(g, lanes=1)
+    t0 = time.time()
+    for p in plans.values():
+        plain_srv.prepare(p)
+    t_warm = time.time() - t0
+    print(f"[bench] warmed {len(plans)} plans in {t_warm:.1f} s", file=sys.stderr, flush=True)
+    plain = []
+    t0 = time.time()
+    for i, key in enumerate(stream):
+        submit(plain_srv, key)
+        r = plain_srv.poll()
+        r.pop("matches", None)
+        plain.append(r)
+        if i % 20 == 0:
+            print(f"[bench] plain {i}/{N}", file=sys.stderr, flush=True)
+    plain_wall = time.time() - t0
+    plain_srv.close()
+    svc = [r["latency_ms"] for r in plain]
+    plain_ok = [i for i in range(N) if plain[i]["status"] == 0]
+    plain_cap = N / (sum(svc) / 1000.0)
+    plain_done_rate = len(plain_ok) / (sum(svc) / 1000.0)
+    qps = args.qps or 0.8 * plain_cap
+    gaps = rng.exponential(1.0 / qps, size=N)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    # plain path under the same arrivals: a FIFO single server with the measured service times
+    fin, plain_lat = 0.0, []
+    for a, sv in zip(arrivals * 1000.0, svc):
+        fin = max(fin, a) + sv
+        plain_lat.append(fin - a)
+
+    srv = B.Server(g, lanes=args.serve_lanes, budget_ms=dl)
+    for p in plans.values():
+        srv.prepare(p)
+    order = {}
+
+    def producer():
+        base = time.time()
+        for i, key in enumerate(stream):
+            wait = base + arrivals[i] - time.time()
+            if wait > 0:
+                time.sleep(wait)
+            order[submit(srv, key)] = i
+
+    t0 = time.time()
+    th = threading.Thread(target=producer)
+    th.start()
+    results = drain(srv, N, "served")
+    th.join()
+    served_wall = time.time() - t0
+    st = srv.stats()
+    srv.close()
+    # capacity: a fresh server (empty prediction cache), the whole stream submitted at once
+    srv = B.Server(g, lanes=args.serve_lanes, budget_ms=dl)
+    for p in plans.values():
+        srv.prepare(p)
+    t0 = time.time()
+    for key in stream:
+        submit(srv, key)
+    burst = drain(srv, N, "burst")
+    burst_wall = time.time() - t0
+    srv.close()
+    # cold: the reference's way, on a few queries of the stream
+    cold = []
+    for key in stream[:args.cold_sample]:
+        t0 = time.time()
+        pp = B.Prepared(g, plans[key])
+        pp.upload(0)
+        pp.count(emit=B.EMIT_NONE, deadline_ms=dl)
+        del pp
+        cold.append(time.time() - t0)
+    print(f"[bench] cold sample {cold}", file=sys.stderr, flush=True)
+
+    by_idx = {order[t]: r for t, r in results.items()}
+    same = all((by_idx[i]["count"], by_idx[i]["hash_sum"]) == (plain[i]["count"], plain[i]["hash_sum"])
+               for i in range(N) if plain[i]["status"] == 0 and by_idx[i]["status"] == 0)
+    done = [by_idx[i] for i in range(N) if by_idx[i]["status"] == 0]
+    lat_done = [r["latency_ms"] for r in done]
+    lat_all = [r["latency_ms"] for r in by_idx.values()]
+    n_status = lambda rs, c: sum(1 for r in rs if r["status"] == c)
+    burst_done = n_status(burst.values(), 0)
+    out = {
+        "metric": "served queries/s",
+        "value": len(done) / served_wall,
+        "unit": "completed queries/s",
+        "n_gpus": 1, "steps": N, "warmup": 0,
+        "ms_per_step": served_wall * 1000.0 / N,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"{name}: {wl['kind']} n={g.n}, seed {wl['seed']}, {g.m // 2} undirected edges, "
+                               f"max degree {g.max_degree}; Poisson arrivals at {qps:.3f} q/s over the 207 "
+                               f"central-node patterns (3-7 vertices), 20% sgm with a {cap} tuple buffer, "
+                               f"per-query deadline {dl} ms",
+                   "queries": N, "qps": qps, "deadline_ms": dl, "lanes": args.serve_lanes},
+        "served": {"p50_ms": pct(lat_done, 50), "p99_ms": pct(lat_done, 99),
+                   "p50_ms_all": pct(lat_all, 50), "p99_ms_all": pct(lat_all, 99),
+                   "completed": len(done), "stopped": n_status(by_idx.values(), B.ETIMEDOUT),
+                   "refused": n_status(by_idx.values(), B.ERANGE),
+                   "failed": sum(1 for r in by_idx.values() if r["status"] not in (0, B.ETIMEDOUT, B.ERANGE)),
+                   "wall_s": served_wall, "peak_running": st["peak_running"], "pilots": st["pilots"],
+                   "pool_peak_bytes": st["pool_peak"]},
+        "plain": {"p50_ms": pct([plain_lat[i] for i in plain_ok], 50),
+                  "p99_ms": pct([plain_lat[i] for i in plain_ok], 99),
+                  "p50_ms_all": pct(plain_lat, 50), "p99_ms_all": pct(plain_lat, 99),
+                  "completed": len(plain_ok), "stopped": n_status(plain, B.ETIMEDOUT),
+                  "queries_per_s": plain_cap, "completed_per_s": plain_done_rate, "wall_s": plain_wall},
+        "capacity": {"served_completed_per_s": burst_done / burst_wall, "served_completed": burst_done,
+                     "served_wall_s": burst_wall, "plain_completed_per_s": plain_done_rate,
+                     "speedup": (burst_done / burst_wall) / plain_done_rate if plain_done_rate else None},
+        "predictor": {"refused_but_plain_completed": sum(1 for i in range(N) if by_idx[i]["status"] == B.ERANGE
+                                                         and plain[i]["status"] == 0),
+                      "admitted_but_stopped": n_status(by_idx.values(), B.ETIMEDOUT)},
+        "cold": {"sample": len(cold), "s_per_query": float(np.mean(cold)) if cold else None,
+                 "what": "prepare (peel + orient + roots) + upload + count + free, per query"},
+        "same_result_alone_and_in_mix": same,
+        "host_s": {"warm_caches": round(t_warm, 2)},
     }
     print(json.dumps(out))
 

@@ -214,16 +214,23 @@ typedef struct {
     int lanes;              /* queries running at once, 1..16 */
     uint64_t pool_bytes;    /* scratch pool; 0 = a quarter of the free HBM */
     uint64_t arena_cap;     /* hub-arena bytes per query; 0 = pool / (2 * lanes) */
+    double budget_ms;       /* > 0: predict each new plan's device time with a pilot run
+                               on a 1/pilot_stride sample of its root queue (deadline
+                               budget_ms / pilot_stride) and refuse, with BEEP_ERANGE,
+                               plans predicted above budget_ms */
+    uint32_t pilot_stride;  /* sample stride of the pilot (0 = 32) */
 } beep_server_opts;
 
 typedef struct {
     uint64_t ticket;
-    int status;             /* 0, BEEP_ETIMEDOUT (deadline passed, partial counts) or an error */
+    int status;             /* 0, BEEP_ETIMEDOUT (deadline passed, partial counts),
+                               BEEP_ERANGE (predicted above the budget, not run) or an error */
     beep_result result;     /* t_enum_ms: device time of the query's launch sequence */
     double queue_ms;        /* submit -> admission */
     double service_ms;      /* admission -> completion */
     double latency_ms;      /* submit -> completion */
     uint64_t footprint;     /* predicted scratch bytes the query was admitted with */
+    double predicted_ms;    /* pilot prediction of the device time (< 0: none) */
 } beep_server_done;
 
 typedef struct {
@@ -231,6 +238,7 @@ typedef struct {
     uint64_t pending, running, peak_running;
     uint64_t pool_bytes, pool_used, pool_peak;
     uint64_t variants, root_queues;
+    uint64_t refused, pilots;   /* refused by the prediction; pilot runs */
 } beep_server_stats;
 
 void beep_server_opts_default(beep_server_opts *o);

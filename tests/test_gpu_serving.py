@@ -78,13 +78,15 @@ def test_server_mix_matches_alone(cl_graph, lanes):
         assert st["peak_running"] > 1  # queries did overlap
 
 
-def test_server_sgm_tuples_match_alone(cl_graph):
+def test_server_sgm_tuples_match_alone():
     say("server sgm tuples")
-    names = ["diamond", "tailed_triangle", "4clique"]
-    srv = B.Server(cl_graph, lanes=3)
+    # 2,222,936 diamonds with 5 hub roots, every tuple fits the buffer
+    g = B.Graph.chung_lu(20000, 40000, 2.1, 2.0, 3)
+    names = ["diamond", "triangle", "4clique"]
+    srv = B.Server(g, lanes=3)
     ref, tickets = {}, {}
     for n in names:
-        a = alone(cl_graph, n, B.SGM, emit=B.EMIT_BUFFER, match_cap=1 << 22)
+        a = alone(g, n, B.SGM, emit=B.EMIT_BUFFER, match_cap=1 << 22)
         assert a["written"] == a["count"]
         ref[n] = a
         tickets[srv.submit(plan_of(n, B.SGM), emit=B.EMIT_BUFFER, match_cap=1 << 22)] = n
@@ -160,3 +162,24 @@ def test_server_deadline(c2_graph):
     ref = alone(c2_graph, "4clique", B.SGM)
     assert out[t_ok]["status"] == 0
     assert (out[t_ok]["count"], out[t_ok]["hash_sum"]) == (ref["count"], ref["hash_sum"])
+
+
+def test_server_prediction_refuses_heavy_plans(c2_graph):
+    say("server prediction chunglu1m")
+    srv = B.Server(c2_graph, lanes=2, budget_ms=50.0)
+    heavy = plan_of("4star", B.SGM)    # sum over roots of C(d, 3) tuples, d up to 29k
+    light = plan_of("triangle", B.SGC)
+    srv.submit(heavy)
+    r1 = srv.poll(timeout_ms=120000)
+    assert r1["status"] == B.ERANGE and r1["predicted_ms"] > 50.0
+    srv.submit(light, emit=B.EMIT_HASH)
+    r2 = srv.poll(timeout_ms=120000)
+    ref = alone(c2_graph, "triangle", B.SGC)
+    assert r2["status"] == 0 and 0.0 <= r2["predicted_ms"] <= 50.0
+    assert (r2["count"], r2["hash_sum"], r2["hash_xor"]) == (ref["count"], ref["hash_sum"], ref["hash_xor"])
+    pilots = srv.stats()["pilots"]
+    srv.submit(heavy)                  # refused from the cached prediction, no new pilot
+    r3 = srv.poll(timeout_ms=120000)
+    st = srv.stats()
+    srv.close()
+    assert r3["status"] == B.ERANGE and st["pilots"] == pilots and st["refused"] == 2
