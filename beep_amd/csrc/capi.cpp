@@ -7,7 +7,9 @@
 #include <sched.h>
 
 #include <algorithm>
-
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
 #include <cstdlib>
 #include <string>
 
@@ -46,6 +48,21 @@ void upload_prep(Prep &pp, int device, int skip_split);
 static thread_local std::string g_last_error;
 
 void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+void progress(const char *fmt, ...)
+{
+    static const bool on = getenv("BEEP_VERBOSE") != nullptr;
+    if (!on) return;
+    static const auto t0 = std::chrono::steady_clock::now();
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "[beep %8.2fs] %s\n", s, buf);
+    fflush(stderr);
+}
 
 unsigned host_threads()
 {

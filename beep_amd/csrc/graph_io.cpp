@@ -282,6 +282,7 @@ static std::unique_ptr<Graph> csr_from_pairs(const std::vector<uint32_t> &a,
                 cnt[b[i]].fetch_add(1, std::memory_order_relaxed);
             }
     }, 1 << 18);
+    progress("csr: %zu pairs over %u vertices counted", S, nmax);
     std::vector<uint64_t> off(nmax + 1, 0);
     for (uint32_t v = 0; v < nmax; v++) off[v + 1] = off[v] + cnt[v].load(std::memory_order_relaxed);
     std::vector<uint32_t> tmp(off[nmax]);
@@ -293,6 +294,7 @@ static std::unique_ptr<Graph> csr_from_pairs(const std::vector<uint32_t> &a,
                 tmp[cnt[b[i]].fetch_add(1, std::memory_order_relaxed)] = a[i];
             }
     }, 1 << 18);
+    progress("csr: %llu entries scattered", (unsigned long long)off[nmax]);
     std::vector<uint64_t> len(nmax, 0);
     parallel_for(nmax, [&](size_t lo, size_t hi) {
         for (size_t v = lo; v < hi; v++) {
@@ -301,6 +303,7 @@ static std::unique_ptr<Graph> csr_from_pairs(const std::vector<uint32_t> &a,
             len[v] = (uint64_t)(std::unique(bgn, end) - bgn);
         }
     }, 1 << 12);
+    progress("csr: rows sorted and deduplicated");
     g->rowptr.assign((size_t)nmax + 1, 0);
     for (uint32_t v = 0; v < nmax; v++) g->rowptr[v + 1] = g->rowptr[v] + len[v];
     g->m = g->rowptr[nmax];
@@ -353,6 +356,7 @@ std::unique_ptr<Graph> gen_rmat(uint32_t scale, uint64_t samples, double pa, dou
             b[i] = (uint32_t)d;
         }
     }, 1 << 16);
+    progress("rmat: %llu samples drawn", (unsigned long long)samples);
     return csr_from_pairs(a, b);
 }
 

@@ -25,6 +25,8 @@ struct Error : std::runtime_error {
 };
 
 void set_last_error(const std::string &msg);
+// One line on stderr per finished preparation step when BEEP_VERBOSE is set.
+void progress(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 
 // ---------------------------------------------------------------- threading
 unsigned host_threads();

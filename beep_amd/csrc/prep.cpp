@@ -88,6 +88,7 @@ int peel(const Graph &g, int min_qdeg, std::vector<uint64_t> &rp, std::vector<ui
         });
         rp.swap(nrp);
         col.swap(ncol);
+        progress("peel: round %d removed the edges of %zu vertices", rounds, S.size());
         if (!((double)S.size() > 0.05 * (double)n)) break;
     }
     return rounds;
@@ -156,6 +157,8 @@ std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc_flag
     uint32_t maxd = 0;
     for (uint32_t v = 0; v < n; v++) maxd = std::max(maxd, deg[v]);
     V.max_degree = maxd;
+    progress("variant (min_qdeg %d, %s): %llu entries oriented", min_qdeg, asc ? "asc" : "desc",
+             (unsigned long long)V.m);
     return vp;
 }
 
@@ -247,6 +250,7 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
         });
     }
     build_tiers(*pp);
+    progress("root queue: %zu roots, %u hubs, %zu LDS batches", pp->roots.size(), pp->n_hub, pp->batches.size());
     return pp;
 }
 

@@ -144,6 +144,7 @@ static void upload_variant(Variant &V, int device)
             }
         });
         dg->vinfo = dev_upload(vi);
+        progress("variant uploaded (%llu entries)", (unsigned long long)V.m);
     } catch (...) {
         delete dg;
         throw;
