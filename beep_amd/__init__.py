@@ -17,7 +17,7 @@ SGM, SGC = 0, 1
 EMIT_NONE, EMIT_HASH, EMIT_BUFFER = 0, 1, 2
 
 __all__ = ["Graph", "Plan", "Prepared", "Batch", "Server", "SG_Match", "SGM", "SGC", "BeepError",
-           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "ETIMEDOUT", "ERANGE", "tuple_hash", "mode_of"]
+           "EMIT_NONE", "EMIT_HASH", "EMIT_BUFFER", "ETIMEDOUT", "ERANGE", "tuple_hash", "mode_of", "convert"]
 
 ETIMEDOUT = -110
 ERANGE = -34
@@ -340,6 +340,14 @@ class SG_Match:
         prep.upload(self.dev)
         self.last = prep.count(device=self.dev, emit=emit)
         return self.last["count"]
+
+
+CONVERSIONS = {"mtx-bel": 1, "tsv-bel": 2, "tsv-mtx": 3, "bel-mtx": 4, "txt-bel": 5}
+
+
+def convert(kind, src, dst):
+    """The reference's -m <kind> -g src -r dst format conversion (MtB_Writer)."""
+    check(lib().beep_convert(CONVERSIONS[kind], str(src).encode(), str(dst).encode()))
 
 
 def tuple_hash(t):

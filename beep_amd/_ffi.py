@@ -75,6 +75,7 @@ SIGNATURES = {
     "beep_graph_describe": (C.c_int, [C.c_void_p, C.POINTER(GraphDesc)]),
     "beep_graph_write_bel": (C.c_int, [C.c_void_p, C.c_char_p]),
     "beep_graph_free": (None, [C.c_void_p]),
+    "beep_convert": (C.c_int, [C.c_int, C.c_char_p, C.c_char_p]),
     "beep_plan_from_mtx": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
     "beep_plan_from_edges": (C.c_int, [u32p, u32p, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
     "beep_plan_describe": (C.c_int, [C.c_void_p, C.POINTER(PlanDesc)]),

@@ -184,6 +184,14 @@ int beep_graph_write_bel(const beep_graph *g, const char *path)
 
 void beep_graph_free(beep_graph *g) { delete g; }
 
+int beep_convert(int kind, const char *src_path, const char *dst_path)
+{
+    return guarded([&] {
+        if (!src_path || !dst_path) throw Error(BEEP_EINVAL, "null path");
+        beep::convert(kind, src_path, dst_path);
+    });
+}
+
 int beep_plan_from_edges(const uint32_t *src, const uint32_t *dst, uint64_t n, int mode, beep_plan **out)
 {
     return guarded([&] {

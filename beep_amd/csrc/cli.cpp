@@ -27,7 +27,9 @@ static void usage()
             "    -g <Src graph FileName>       Name of file with input graph (.bel, .tsv, .mtx)\n"
             "    -t <Pattern graph filename>   Query graph (.mtx, 0-based, both directions listed)\n"
             "    -d <Device Id>                HIP device id (default = 0)\n"
-            "    -m <MainTask>                 sgm (enumerate) or sgc (count)\n"
+            "    -m <MainTask>                 sgm (enumerate) or sgc (count); or a conversion\n"
+            "                                  mtx-bel, tsv-bel, tsv-mtx, bel-mtx, txt-bel (-g -> -r)\n"
+            "    -r <Dst graph FileName>       Output of a conversion\n"
             "    -s <sort>                     Sort read edges by src then dst\n"
             "    -n <num devices>              Only 1 is supported\n"
             "    -h                            Help\n"
@@ -41,7 +43,8 @@ static double now_s()
 
 int main(int argc, char **argv)
 {
-    const char *gpath = nullptr, *tpath = nullptr;
+    const char *gpath = nullptr, *tpath = nullptr, *rpath = nullptr;
+    int conv = 0;
     int dev = 0, ndev = 1, sort_edges = 0;
     int mode = -1;
     const char *task = "kc";
@@ -55,6 +58,11 @@ int main(int argc, char **argv)
             task = optarg;
             if (!strcmp(optarg, "sgm")) mode = BEEP_MODE_SGM;
             else if (!strcmp(optarg, "sgc")) mode = BEEP_MODE_SGC;
+            else if (!strcmp(optarg, "mtx-bel")) conv = BEEP_CONV_MTX_BEL;
+            else if (!strcmp(optarg, "tsv-bel")) conv = BEEP_CONV_TSV_BEL;
+            else if (!strcmp(optarg, "tsv-mtx")) conv = BEEP_CONV_TSV_MTX;
+            else if (!strcmp(optarg, "bel-mtx")) conv = BEEP_CONV_BEL_MTX;
+            else if (!strcmp(optarg, "txt-bel")) conv = BEEP_CONV_TXT_BEL;
             else if (!strcmp(optarg, "tc") || !strcmp(optarg, "kcore") || !strcmp(optarg, "ktruss") ||
                      !strcmp(optarg, "kc") || !strcmp(optarg, "lkc") || !strcmp(optarg, "mc") ||
                      !strcmp(optarg, "cd"))
@@ -73,7 +81,8 @@ int main(int argc, char **argv)
             fprintf(stderr, "\nUnrecognized option!\n");
             usage();
             return 0;
-        case 'r': case 'x': case 'o': case 'a': case 'v': case 'k': case 'p': case 'e': case 'q':
+        case 'r': rpath = optarg; break;
+        case 'x': case 'o': case 'a': case 'v': case 'k': case 'p': case 'e': case 'q':
         case 'w': case 'c': break;
         default:  // the reference prints usage and exits with status 0 (Config.h:421-424)
             fprintf(stderr, "\nUnrecognized option!\n");
@@ -90,6 +99,15 @@ int main(int argc, char **argv)
     printf("    Pattern Graph: %s\n", tpath ? tpath : "");
     printf("    Device Id = %d\n", dev);
     printf("    Main Task = %s\n", task);
+    if (conv) {  // format conversion -g -> -r, nothing else (main.cu:40-68)
+        printf("    DST Graph: %s\n", rpath ? rpath : "");
+        if (!gpath || !rpath) { usage(); return 2; }
+        if (beep_convert(conv, gpath, rpath)) {
+            fprintf(stderr, "beep: %s\n", beep_last_error());
+            return 1;
+        }
+        return 0;
+    }
     if (!gpath) { usage(); return 2; }
 
     double t0 = now_s();

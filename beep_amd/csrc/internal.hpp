@@ -77,6 +77,8 @@ void read_edges(const std::string &path, std::vector<uint32_t> &src, std::vector
 // cutoff = ceil(mean + 6 sd) (utils.cuh:186-261).
 uint32_t compute_cutoff(const Graph &g, double *mean, double *sd);
 void write_bel(const Graph &g, const std::string &path);
+// Format converters (BEEP_CONV_*), convert.cpp.
+void convert(int kind, const std::string &src, const std::string &dst);
 
 std::unique_ptr<Graph> gen_rmat(uint32_t scale, uint64_t samples, double a, double b, double c,
                                 uint64_t seed, uint32_t n_keep);

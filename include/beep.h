@@ -91,6 +91,17 @@ int beep_graph_describe(const beep_graph *g, beep_graph_desc *out);
 int beep_graph_write_bel(const beep_graph *g, const char *path);
 void beep_graph_free(beep_graph *g);
 
+/* Format converters, -m mtx-bel | tsv-bel | tsv-mtx | bel-mtx | txt-bel
+ * (main.cu:40-68; MtB_Writer, FIleReader.cuh:459-912), byte for byte as the
+ * reference writes them.  Replaces MtB_Writer::write_market_bel /
+ * write_tsv_bel / write_tsv_market / write_bel_market / write_txt_bel. */
+#define BEEP_CONV_MTX_BEL 1
+#define BEEP_CONV_TSV_BEL 2
+#define BEEP_CONV_TSV_MTX 3
+#define BEEP_CONV_BEL_MTX 4
+#define BEEP_CONV_TXT_BEL 5
+int beep_convert(int kind, const char *src_path, const char *dst_path);
+
 /* ------------------------------------------------------------------------ */
 /* Query plan.  Replaces SG_Match::preprocess_query / detect_symmetry /       */
 /* detect_reuse / check_unmat (subgraph_matching.cuh:257-631), with its own   */
