@@ -4,15 +4,18 @@
 //   beep -g <data.bel|.tsv|.mtx> -t <query.mtx> -m sgm|sgc [-d dev] [-s 1]
 //
 // Output ends with the reference's "------------- Counter = N" line
-// (subgraph_matching.cuh:1277).  Flags the path does not use (-r -w -x -o -a
-// -v -k -p -e -q -c) are accepted and ignored; -c is overwritten by the
-// computed cutoff in the reference as well (main.cu:132).  -n > 1 is refused:
-// this build drives one GPU.
+// (subgraph_matching.cuh:1277).  -m mtx-bel | tsv-bel | tsv-mtx | bel-mtx |
+// txt-bel convert -g into -r; -x prints the statistics block.  Flags the path
+// does not use (-w -o -a -v -k -p -e -q -c) are accepted and ignored; -c is
+// overwritten by the computed cutoff in the reference as well (main.cu:132).
+// -n > 1 is refused: this build drives one GPU.
 
 #include <unistd.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
+#include <map>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -36,6 +39,82 @@ static void usage()
             "  Environment: BEEP_EMIT=none|hash (default none), BEEP_LANES=<1..64>\n\n");
 }
 
+// -x: the reference's statistics block (main.cu:135-170): MatrixStats and
+// PrintMtarixStruct (utils.cuh:110-183, 264-306) and the bitmap-encoding
+// element counts, with its 32-bit unsigned arithmetic and float cell
+// counters.  Its trailing single-row intersection prints nothing and is not
+// repeated; where it divides by zero (fewer than 10 rows) or indexes past its
+// 10 x 10 grid (the last rows when n is not a multiple of 10) this build uses
+// one row per cell and clamps the cell index instead.
+static void print_stats(const beep_graph_desc &gd)
+{
+    typedef unsigned int T;
+    const T n = gd.n, nnz = (T)gd.m;
+    std::map<T, T> dict;
+    for (T i = 0; i < n; i++) dict[(T)(gd.rowptr[i + 1] - gd.rowptr[i])]++;
+    T minRow = 0, maxRow = 1, totalRow = 0, median = 0, countMin = 0, countMax = 0, countMedian = 0;
+    for (const auto &kv : dict) {
+        if (kv.first == 0) continue;
+        if (minRow == 0) {
+            minRow = kv.first;
+            countMin = kv.second;
+        }
+        if (countMedian < kv.second) {
+            median = kv.first;
+            countMedian = kv.second;
+        }
+        totalRow += kv.second * kv.first;
+        maxRow = kv.first;
+        countMax = kv.second;
+    }
+    const double mean = 1.0 * totalRow / n;
+    double sd = 0;
+    for (T i = 0; i < n; i++) {
+        const double diff = (T)(gd.rowptr[i + 1] - gd.rowptr[i]) - mean;
+        sd += diff * diff;
+    }
+    sd = std::sqrt(sd / n);
+    printf("%u,%u,%u,%u,%.1f,%.1f,%u,%u,%u,%u,%u\n", nnz, n, n, minRow, mean, sd, median, maxRow, countMin,
+           countMedian, countMax);
+
+    const int res = 10;
+    float st[res][res] = {};
+    T unit = n / res;
+    if (unit == 0) unit = 1;
+    for (T i = 0; i < n; i++) {
+        T ri = i / unit;
+        if (ri >= (T)res) ri = res - 1;
+        for (uint64_t j = gd.rowptr[i]; j < gd.rowptr[i + 1]; j++) {
+            T ci = gd.col[j] / unit;
+            if (ci >= (T)res) ci = res - 1;
+            st[ri][ci]++;
+        }
+    }
+    for (int i = 0; i < res; i++) {
+        for (int j = 0; j < res; j++) {
+            const float perc = st[i][j] / nnz;
+            if (perc >= 0.001) printf("%d,", (int)(1000 * st[i][j] / nnz));
+            else if (perc > 0) printf("+,");
+            else printf("-,");
+        }
+        printf("\n");
+    }
+
+    printf("Now # of bytes we need to make this matrix binary encoded !!\n");
+    const T dv = 32;
+    uint64_t sum = 0, sumc = 0;
+    for (T i = 0; i < n; i++) {
+        const T deg = (T)(gd.rowptr[i + 1] - gd.rowptr[i]);
+        sum += (uint64_t)(T)(deg * (deg + dv - 1) / dv);
+        const uint64_t nelem8 = deg / dv;
+        const uint64_t rem = deg - nelem8 * dv;
+        sumc += dv * nelem8 * (1 + nelem8) / 2;
+        sumc += rem * (1 + nelem8);
+    }
+    printf("n = %u, m = %u, elements = %llu\n", n, nnz, (unsigned long long)sum);
+    printf("n = %u, m = %u, elements = %llu\n", n, nnz, (unsigned long long)sumc);
+}
+
 static double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -44,7 +123,7 @@ static double now_s()
 int main(int argc, char **argv)
 {
     const char *gpath = nullptr, *tpath = nullptr, *rpath = nullptr;
-    int conv = 0;
+    int conv = 0, stats = 0;
     int dev = 0, ndev = 1, sort_edges = 0;
     int mode = -1;
     const char *task = "kc";
@@ -82,7 +161,8 @@ int main(int argc, char **argv)
             usage();
             return 0;
         case 'r': rpath = optarg; break;
-        case 'x': case 'o': case 'a': case 'v': case 'k': case 'p': case 'e': case 'q':
+        case 'x': stats = 1; break;
+        case 'o': case 'a': case 'v': case 'k': case 'p': case 'e': case 'q':
         case 'w': case 'c': break;
         default:  // the reference prints usage and exits with status 0 (Config.h:421-424)
             fprintf(stderr, "\nUnrecognized option!\n");
@@ -120,6 +200,7 @@ int main(int argc, char **argv)
     beep_graph_describe(g, &gd);
     printf("Read graph time: %f s\n", now_s() - t0);
     printf("%llu,%u,%u,%.1f,%.1f,%u\n", (unsigned long long)gd.m, gd.n, gd.n, gd.mean, gd.sd, gd.max_degree);
+    if (stats) print_stats(gd);
     if (mode < 0) {  // tasks other than sgm/sgc only load the graph in the reference
         beep_graph_free(g);
         return 0;

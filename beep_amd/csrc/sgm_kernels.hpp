@@ -2,6 +2,8 @@
 // host engine.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace beep {
@@ -82,6 +84,14 @@ struct HubQueue {
     uint32_t split_min;             // donate when |C_2| > split_min
     uint32_t chunk;                 // candidates per record
 };
+
+// Words (u64) of one hub-DFS wave's buffer: nlev candidate-stack rows, the
+// bitmap row of x_1, and the compacted word indices (u32) of the last level's
+// fixed part.
+__host__ __device__ inline uint64_t hub_wave_words(int nlev, uint32_t wmax)
+{
+    return (uint64_t)(nlev + 1) * wmax + (wmax + 1) / 2;
+}
 
 struct KOut {
     unsigned long long *acc;  // [0] count [1] hash_sum [2] hash_xor [3] written

@@ -103,3 +103,25 @@ def test_cli_conversion(tmp_path):
     r = subprocess.run([CLI, "-m", "txt-bel", "-g", str(src), "-r", str(dst)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert dst.read_bytes() == bel((2, 1, 0), (1, 2, 0))
+
+
+def test_cli_statistics_block(tmp_path):
+    """-x on a 20-vertex star (centre 0): every figure worked out by hand from
+    MatrixStats / PrintMtarixStruct (utils.cuh:110-183, 264-306) and the
+    encoding counts of main.cu:140-170 (32-bit words: 19*50/32 = 29 for the
+    centre plus 1 per leaf; the triangular layout needs 19 + 19)."""
+    edges = [(0, i) for i in range(1, 20)]
+    pairs = sorted(edges + [(b, a) for a, b in edges])
+    g = B.Graph.from_edges([a for a, _ in pairs], [b for _, b in pairs])
+    path = tmp_path / "star.bel"
+    g.write_bel(path)
+    r = subprocess.run([CLI, "-g", str(path), "-m", "kc", "-x", "1"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    i = lines.index("38,20,20,1,1.9,3.9,1,19,19,19,1")
+    grid = lines[i + 1:i + 11]
+    assert grid[0] == "52," * 10
+    assert all(row == "52," + "-," * 9 for row in grid[1:])
+    assert lines[i + 11] == "Now # of bytes we need to make this matrix binary encoded !!"
+    assert lines[i + 12] == "n = 20, m = 38, elements = 48"
+    assert lines[i + 13] == "n = 20, m = 38, elements = 38"
