@@ -175,10 +175,11 @@ class Prepared:
         check(lib().beep_prep_upload(self._h, device))
 
     def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0,
-              deadline_ms=0.0):
+              deadline_ms=0.0, profile=False):
         """Runs the enumeration.  With a deadline the result may be partial:
-        then ``stopped`` is 1 (the C call's BEEP_ETIMEDOUT is not raised)."""
-        o = _opts(device, emit, lanes, donate, donate_min, deadline_ms)
+        then ``stopped`` is 1 (the C call's BEEP_ETIMEDOUT is not raised).
+        With ``profile`` the balance figures are filled in."""
+        o = _opts(device, emit, lanes, donate, donate_min, deadline_ms, profile)
         buf = None
         if emit == EMIT_BUFFER and match_cap:
             buf = np.zeros(match_cap * self.plan.width, dtype=np.uint32)
@@ -200,7 +201,7 @@ class Prepared:
             self._h = C.c_void_p()
 
 
-def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline_ms=0.0):
+def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline_ms=0.0, profile=False):
     o = _ffi.Opts()
     lib().beep_opts_default(C.byref(o))
     o.device, o.emit, o.lanes = device, emit, lanes
@@ -208,6 +209,7 @@ def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline
     if donate_min:
         o.donate_min = donate_min
     o.deadline_ms = float(deadline_ms)
+    o.profile = 1 if profile else 0
     return o
 
 
@@ -215,7 +217,8 @@ def _result_dict(r):
     return dict(count=r.count, hash_sum=r.hash_sum, hash_xor=r.hash_xor,
                 emitted=r.n_emitted, written=r.n_written, t_enum_ms=r.t_enum_ms,
                 t_hub_ms=r.t_hub_ms, t_lds_ms=r.t_lds_ms, donations=r.donations,
-                stopped=r.stopped)
+                stopped=r.stopped, balance_lds=r.balance_lds, balance_hub=r.balance_hub,
+                donated_frac=r.donated_frac)
 
 
 class Batch:

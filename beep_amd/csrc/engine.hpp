@@ -60,7 +60,7 @@ struct DevRoots {
 // Scratch sizes of one launch sequence.
 struct ScratchNeeds {
     uint64_t arena_words = 0, hash_slots = 0, stk_words = 0;
-    uint64_t queue_cap = 0, slot_cap = 0, dbuf_words = 0;
+    uint64_t queue_cap = 0, slot_cap = 0, dbuf_words = 0, prof_words = 0;
     uint64_t bytes() const;
 };
 
@@ -73,6 +73,7 @@ struct Scratch {
     uint64_t *slots = nullptr;
     uint32_t *dbuf = nullptr;
     unsigned long long *ctr = nullptr;
+    unsigned long long *prof = nullptr;
     ScratchNeeds cap;
 };
 void carve(Scratch &s, char *base, const ScratchNeeds &n);
@@ -131,5 +132,7 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
                    unsigned long long *host_ctr);
 void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t st);
 void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res);
+// Balance figures from the busy-time accumulators (kProfPhases x kProfSlots).
+void fill_profile(const unsigned long long *prof, beep_result &res);
 
 }  // namespace beep

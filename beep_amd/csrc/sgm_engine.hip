@@ -279,7 +279,7 @@ uint64_t ScratchNeeds::bytes() const
 {
     return align256(kCtrWords * 8) + align256(arena_words * 8) + 2 * align256(hash_slots * 4) +
            align256(stk_words * 8) + align256(queue_cap * sizeof(HubChunk)) + align256(slot_cap * 8) +
-           align256(dbuf_words * 4);
+           align256(dbuf_words * 4) + align256(prof_words * 8);
 }
 
 void carve(Scratch &s, char *base, const ScratchNeeds &n)
@@ -298,6 +298,7 @@ void carve(Scratch &s, char *base, const ScratchNeeds &n)
     s.queue = n.queue_cap ? (HubChunk *)take(n.queue_cap * sizeof(HubChunk)) : nullptr;
     s.slots = n.slot_cap ? (uint64_t *)take(n.slot_cap * 8) : nullptr;
     s.dbuf = n.dbuf_words ? (uint32_t *)take(n.dbuf_words * 4) : nullptr;
+    s.prof = n.prof_words ? (unsigned long long *)take(n.prof_words * 8) : nullptr;
     s.cap = n;
 }
 
@@ -326,6 +327,7 @@ ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, in
         n.slot_cap = std::max<uint64_t>(1, std::min<uint64_t>(1ull << 26, R.don_words));
     }
     if (o.emit == BEEP_EMIT_BUFFER && o.match_cap) n.dbuf_words = o.match_cap * (uint64_t)p.L;
+    if (o.profile) n.prof_words = (uint64_t)kProfPhases * kProfSlots;
     return n;
 }
 
@@ -354,7 +356,7 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     LaunchCfg c{};
     const bool buf = o.emit == BEEP_EMIT_BUFFER && o.match_cap && s.dbuf;
     c.out = KOut{s.ctr + kCtrAcc, buf ? s.dbuf : nullptr, buf ? o.match_cap : 0, s.ctr + kCtrDeadline,
-                 s.ctr + kCtrStop};
+                 s.ctr + kCtrStop, s.prof};
     c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
     c.kg = KGraph{V.dev->ocol, V.dev->vinfo};
     c.nlev = p.L > 3 ? p.L - 3 : 1;
@@ -443,6 +445,7 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
 {
     hipStream_t st = lane.main;
     HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, st));
+    if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, st));
     HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, st));
     HIP_CHECK(hipEventRecord(lane.ev[0], st));
     // The LDS lane classes run on the side streams, concurrently with the hub
@@ -467,6 +470,33 @@ void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result 
     res.n_written = (o.emit == BEEP_EMIT_BUFFER && o.match_cap) ? std::min<uint64_t>(acc[3], o.match_cap) : 0;
     res.donations = acc[kCtrDonated];
     res.stopped = acc[kCtrStop] ? 1u : 0u;
+}
+
+void fill_profile(const unsigned long long *prof, beep_result &res)
+{
+    auto balance = [](const std::vector<double> &b) {
+        double sum = 0, mx = 0;
+        size_t n = 0;
+        for (double x : b)
+            if (x > 0) {
+                sum += x;
+                mx = std::max(mx, x);
+                n++;
+            }
+        return n && mx > 0 ? sum / n / mx : 0.0;
+    };
+    std::vector<double> lds(kProfSlots), hub(kProfSlots);
+    double own = 0, don = 0;
+    for (int i = 0; i < kProfSlots; i++) {
+        lds[i] = (double)prof[i];
+        const double a = (double)prof[kProfSlots + i], b = (double)prof[2 * kProfSlots + i];
+        hub[i] = a + b;
+        own += a;
+        don += b;
+    }
+    res.balance_lds = balance(lds);
+    res.balance_hub = balance(hub);
+    res.donated_frac = (own + don) > 0 ? don / (own + don) : 0.0;
 }
 
 // Sizes (grow-only) and carves the owned scratch region of a single query.
@@ -509,6 +539,11 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     if (c.out.buf && o.match_buf && res.n_written)
         HIP_CHECK(hipMemcpy(o.match_buf, c.out.buf, res.n_written * (uint64_t)p.L * sizeof(uint32_t),
                             hipMemcpyDeviceToHost));
+    if (d.scratch.prof) {
+        std::vector<unsigned long long> prof((size_t)kProfPhases * kProfSlots);
+        HIP_CHECK(hipMemcpy(prof.data(), d.scratch.prof, prof.size() * 8, hipMemcpyDeviceToHost));
+        fill_profile(prof.data(), res);
+    }
 }
 
 void upload_prep(Prep &pp, int device, int skip_split) { upload(pp, device, skip_split); }
@@ -630,6 +665,7 @@ static void batch_capture(Batch &b)
     for (int i = 0; i < n; i++) {
         const Scratch &s = b.scr[i];
         HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, b.s0));
+        if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, b.s0));
         HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, b.cfgs[i].deadline_ticks, b.s0));
     }
     HIP_CHECK(launch_fill_zero(b.cursor, 4, b.s0));

@@ -114,6 +114,13 @@ __device__ __forceinline__ void mark_stop(const KOut &out) { atomicOr(out.stop, 
 // DFS steps between two deadline checks.
 constexpr uint32_t kClockEvery = 1024;
 
+__device__ __forceinline__ void prof_add(const KOut &out, int phase, uint32_t slot, uint64_t t0)
+{
+    if (out.prof)
+        atomicAdd(&out.prof[phase * kProfSlots + (slot % kProfSlots)],
+                  (unsigned long long)((uint64_t)wall_clock64() - t0));
+}
+
 // Flushes per-lane accumulators: one atomic per wave and counter.
 __device__ __forceinline__ void flush_acc(const KOut &out, uint64_t cnt, uint64_t hs, uint64_t hx)
 {
@@ -447,6 +454,7 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
     }
     uint64_t cnt = 0, hs = 0, hx = 0;
     const uint64_t dl = *out.dl;
+    const uint64_t t_start = wall_clock64();
     for (;;) {
         if (tid == 0) {
             if (past(dl)) {
@@ -462,6 +470,7 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
         process_batch<P>(g, pl, v, recs, batch_first[b], batch_count[b], out, dl, P, cnt, hs, hx);
     }
     flush_acc(out, cnt, hs, hx);
+    if (tid == 0) prof_add(out, 0, blockIdx.x, t_start);
 }
 
 // One persistent launch over the LDS batches of several prepared queries.
@@ -491,9 +500,11 @@ __global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__r
         __syncthreads();
         uint64_t cnt = 0, hs = 0, hx = 0;
         const uint64_t dl = *q.out.dl;
+        const uint64_t t_start = wall_clock64();
         process_batch<0>(q.g, pl, v, q.recs, q.batch_first[w.batch], q.batch_count[w.batch], q.out, dl,
                          1 << w.lanes_log2, cnt, hs, hx);
         flush_acc(q.out, cnt, hs, hx);
+        if (tid == 0) prof_add(q.out, 0, blockIdx.x, t_start);
     }
 }
 
@@ -647,6 +658,8 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     uint32_t nf = 0;  // entries of fidx (wave-uniform)
     uint64_t cnt = 0, hs = 0, hx = 0;
     const uint64_t dl = *out.dl;
+    const uint64_t t_start = wall_clock64();
+    const uint32_t wslot = blockIdx.x * (kBlock / 64) + wv;
     uint32_t spins = 0;
     auto xget = [&](int q) -> uint32_t { return sX[wv][q]; };
 
@@ -852,6 +865,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             dfs3(ch.lo, ch.hi);
         }
         flush_acc(out, cnt, hs, hx);
+        if (lane == 0) prof_add(out, 2, wslot, t_start);
         return;
     }
 
@@ -946,6 +960,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         dfs3(0, 0xffffffffu);
     }
     flush_acc(out, cnt, hs, hx);
+    if (lane == 0) prof_add(out, 1, wslot, t_start);
 }
 
 // ------------------------------------------------------------ launchers

@@ -102,7 +102,13 @@ struct KOut {
     // leave their DFS once it has passed, and set *stop.
     const unsigned long long *dl;
     unsigned long long *stop;
+    // Optional busy-time accumulators (wall_clock64 ticks), kProfPhases x
+    // kProfSlots: phase 0 LDS workgroups, 1 hub waves on their own items, 2 hub
+    // waves on donated subtrees; slot = workgroup (LDS) or wave (hub) index.
+    unsigned long long *prof;
 };
+
+constexpr int kProfSlots = 4096, kProfPhases = 3;
 
 // One prepared query of a multi-query launch.
 struct MultiQuery {

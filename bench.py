@@ -165,6 +165,12 @@ def main():
             per_query_lds[i] += r["t_lds_ms"]
             counts[i] = r["count"]
     wall = time.time() - wall0
+    # one profiled run per query (untimed): busy-time balance of workgroups / waves
+    balance = {}
+    for (q, _), (_, _, _, pp) in zip(queries, preps):
+        r = pp.count(emit=B.EMIT_NONE, lanes=args.lanes, profile=True)
+        balance[q] = {"lds": round(r["balance_lds"], 3), "hub": round(r["balance_hub"], 3),
+                      "donated_frac": round(r["donated_frac"], 3)}
     K = args.steps
     dev_ms_step = sum(per_query_ms) / K
     matches = sum(counts)
@@ -230,6 +236,7 @@ def main():
         "per_query_ms": {q: per_query_ms[i] / K for i, (q, _) in enumerate(queries)},
         "per_query_phase_ms": {q: {"hub": per_query_hub[i] / K, "lds": per_query_lds[i] / K}
                                for i, (q, _) in enumerate(queries)},
+        "balance": balance,
         "host_s": {"generate": round(t_gen, 3), "prepare": round(t_prep, 3),
                    "upload": round(t_upload, 3)},
     }
@@ -519,9 +526,11 @@ def run_large(args):
             pp = B.Prepared(g, B.Plan.from_edges(s, d, B.mode_of(task)))
             pp.upload(0)
             rec["prepare_s"] = round(time.time() - t0, 2)
-            r = pp.count(deadline_ms=dl)
+            r = pp.count(deadline_ms=dl, profile=True)
             rec.update(count=r["count"], t_enum_ms=r["t_enum_ms"], hub_ms=r["t_hub_ms"], lds_ms=r["t_lds_ms"],
                        donations=r["donations"], stopped=r["stopped"], alg_bytes=pp.alg_bytes,
+                       balance_lds=r["balance_lds"], balance_hub=r["balance_hub"],
+                       donated_frac=r["donated_frac"],
                        achieved_gbs=pp.alg_bytes / (r["t_enum_ms"] / 1000.0) / 1e9,
                        matches_per_s=r["count"] / (r["t_enum_ms"] / 1000.0))
             rec["frac"] = rec["achieved_gbs"] / PEAK_HBM_GBS

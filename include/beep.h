@@ -174,6 +174,7 @@ typedef struct {
     int timing;             /* 1 = record HIP events around the enumeration */
     double deadline_ms;     /* > 0: stop the enumeration this long after it starts on
                                the device; the result is then partial and marked stopped */
+    int profile;            /* 1 = busy-time balance of workgroups / waves (beep_count) */
 } beep_opts;
 
 typedef struct {
@@ -184,6 +185,11 @@ typedef struct {
     double t_hub_ms, t_lds_ms; /* device time of the hub and LDS phases */
     uint64_t donations;     /* subtrees donated through the ring */
     uint64_t stopped;       /* 1: the deadline passed; counts are partial */
+    /* with opts.profile (the reference's TIMER build, sgm_kernelsLD_timer.cuh:7-64,
+     * 339-349, as busy-time balance): mean / max busy time over the LDS
+     * workgroups and over the hub-DFS waves (1 = perfectly even), and the
+     * share of hub-DFS busy time spent on donated subtrees */
+    double balance_lds, balance_hub, donated_frac;
 } beep_result;
 
 void beep_opts_default(beep_opts *o);

@@ -245,3 +245,19 @@ def test_batch_graph_matches_single_queries(cl_graph, mode):
     s, d = P.directed(P.read_mtx_edges("diamond"))
     o = O.Prepared(og, O.plan(s, d, mode)).count(want_hash=True)
     assert single[2]["count"] == o["count"]
+
+
+@pytest.mark.parametrize("name,mode", [("w5", B.SGM), ("diamond", B.SGM), ("4clique", B.SGC)])
+def test_profiled_run_counts_the_same(cl_graph, name, mode):
+    g, og = cl_graph
+    say("profile", name, mode)
+    s, d = P.directed(P.read_mtx_edges(name))
+    pp = B.Prepared(g, B.Plan.from_edges(s, d, mode))
+    pp.upload(0)
+    a = pp.count(emit=B.EMIT_HASH)
+    b = pp.count(emit=B.EMIT_HASH, profile=True)
+    assert (a["count"], a["hash_sum"], a["hash_xor"]) == (b["count"], b["hash_sum"], b["hash_xor"])
+    assert 0.0 < b["balance_lds"] <= 1.0
+    assert 0.0 <= b["balance_hub"] <= 1.0 and 0.0 <= b["donated_frac"] <= 1.0
+    if pp.arrays()["n_hub_roots"]:
+        assert b["balance_hub"] > 0.0
