@@ -233,8 +233,10 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
         if (pp->reduced)
             std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return len_of(a) > len_of(b); });
         if (sample_stride > 1) {
+            // the middle of every stride: the heaviest root alone does not
+            // stand for the stride-1 roots after it
             size_t o = 0;
-            for (size_t i = 0; i < idx.size(); i += sample_stride) idx[o++] = idx[i];
+            for (size_t i = sample_stride / 2; i < idx.size(); i += sample_stride) idx[o++] = idx[i];
             idx.resize(o);
         }
         pp->wu.resize(idx.size());

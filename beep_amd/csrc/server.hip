@@ -13,10 +13,10 @@
 // others run are started right away and share the GPU with them.  A query
 // can carry a deadline after which its kernels stop (the result is then
 // marked stopped).  With a budget, each new plan first runs as a pilot on a
-// stratified 1/stride sample of its root queue (deadline budget/stride, on
-// the lane the query was admitted to); the pilot's device time times the
-// stride predicts the full run, and plans predicted above the budget are
-// refused without running.  Predictions are cached per plan.
+// stratified 1/stride sample of its root queue (deadline 2 x budget/stride,
+// on the lane the query was admitted to); the pilot's device time times the
+// stride predicts the full run, and plans predicted above twice the budget
+// are refused without running.  Predictions are cached per plan.
 
 #include <hip/hip_runtime.h>
 
@@ -33,6 +33,13 @@
 namespace beep {
 
 using Clock = std::chrono::steady_clock;
+
+// A plan is refused when its predicted device time exceeds kPilotSlack x the
+// budget (the pilot's deadline scales the same way).  Sampling error and the
+// pilot sharing the GPU with running queries make predictions noisy; refusing
+// a query that would finish costs a completion, admitting one that would not
+// costs at most its deadline.
+constexpr double kPilotSlack = 2.0;
 
 static double ms_between(Clock::time_point a, Clock::time_point b)
 {
@@ -196,7 +203,7 @@ beep_opts Server::pilot_opts(const beep_opts &o) const
     po.emit = BEEP_EMIT_NONE;
     po.match_cap = 0;
     po.match_buf = nullptr;
-    po.deadline_ms = budget_ms / stride;
+    po.deadline_ms = kPilotSlack * budget_ms / stride;
     return po;
 }
 
@@ -315,7 +322,7 @@ void Server::run()
                 predicted[sl.job.sig] = pred;
                 sl.job.predicted_ms = pred;
                 sl.job.piloting = false;
-                if (pred > budget_ms) {
+                if (pred > kPilotSlack * budget_ms) {
                     finish(sl, BEEP_ERANGE, "predicted device time above the budget");
                     continue;
                 }
@@ -341,7 +348,7 @@ void Server::run()
                         auto it = predicted.find(j.sig);
                         if (it != predicted.end()) {
                             j.predicted_ms = it->second;
-                            if (j.predicted_ms > budget_ms)
+                            if (j.predicted_ms > kPilotSlack * budget_ms)
                                 throw Error(BEEP_ERANGE, "predicted device time above the budget");
                         } else {
                             j.pilot = get_prep(j.plan, stride);

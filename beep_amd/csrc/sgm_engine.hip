@@ -317,8 +317,7 @@ ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, in
     }
     const int nlev = p.L > 3 ? p.L - 3 : 1;
     if (R.nhubs) {
-        // per wave: nlev stack rows, the row of x_1 and the last level's word
-        // list (used when they do not fit LDS)
+        // per wave: nlev stack rows + the row of x_1 (used when they do not fit LDS)
         n.stk_words = (uint64_t)hub_grid_of(R, nlev, cus) * (kBlock / 64) *
                       hub_wave_words(nlev, std::max<uint32_t>(R.wmax, 1));
     }

@@ -654,8 +654,6 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     if (LDSBUF) my = hsm + (uint64_t)wv * wave_words;
     else my = stk + ((uint64_t)blockIdx.x * (kBlock / 64) + wv) * wave_words;
     uint64_t *e1 = my + (uint64_t)nlev * wmax;
-    uint32_t *fidx = (uint32_t *)(my + (uint64_t)(nlev + 1) * wmax);
-    uint32_t nf = 0;  // entries of fidx (wave-uniform)
     uint64_t cnt = 0, hs = 0, hx = 0;
     const uint64_t dl = *out.dl;
     const uint64_t t_start = wall_clock64();
@@ -699,79 +697,10 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         return t;
     };
 
-    // The last level L-1 splits into a part fixed once x_1..x_{L-3} are chosen
-    // (rows of earlier positions, exclusions, symmetry bounds) and the part of
-    // x_{L-2}.  Only words where the fixed part is non-zero can hold
-    // candidates; their indices are compacted into fidx when x_{L-3} changes
-    // and the count level visits those words only.
-    const int lc = L - 1, lv = L - 2;
-    const uint32_t fq = sQM[lc] & ~(1u << lv), fs = sSM[lc] & ~(1u << lv);
-    const bool vq = (sQM[lc] >> lv) & 1u, vs = (sSM[lc] >> lv) & 1u;
-    auto fixed_word = [&](uint32_t w) -> uint64_t {
-        uint64_t t = valid_mask(w, d);
-        uint32_t qm = fq;
-        while (qm) {
-            const int q = __builtin_ctz(qm);
-            qm &= qm - 1;
-            if (q == 1 && row1) t &= e1[w];
-            else t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
-        }
-        for (int s2 = 1; s2 < lv; s2++) t &= ~bit_at(xget(s2), w);
-        uint32_t sm = fs;
-        while (sm) {
-            const int s2 = __builtin_ctz(sm);
-            sm &= sm - 1;
-            t &= gt_mask(xget(s2), w);
-        }
-        return t;
-    };
-    auto build_fixed = [&]() {
-        uint32_t lo = 0;
-        uint32_t sm = fs;
-        while (sm) {
-            const int s2 = __builtin_ctz(sm);
-            sm &= sm - 1;
-            const uint32_t v = xget(s2) >> 6;
-            lo = v > lo ? v : lo;
-        }
-        uint32_t n = 0;
-        for (uint32_t rr = lo >> 6; rr < R; rr++) {
-            const uint32_t w = rr * 64 + lane;
-            const bool nz = w < W && w >= lo && fixed_word(w) != 0ull;
-            const uint64_t bal = __ballot(nz);
-            if (nz) fidx[n + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = w;
-            n += (uint32_t)__popcll(bal);
-        }
-        nf = n;
-        // the count level reads fidx and e1 words written by other lanes of
-        // this wave (in LDS or, for very wide hubs, in HBM)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    };
-    // count level: x_{L-2} = xget(lv) chosen, words >= lo
-    auto count_last = [&](uint32_t lo) {
-        uint32_t a = 0, b = nf;  // first entry with index >= lo
-        while (a < b) {
-            const uint32_t m = (a + b) >> 1;
-            if (fidx[m] < lo) a = m + 1; else b = m;
-        }
-        const uint32_t xv = xget(lv);
-        for (uint32_t i = a + lane; i < nf; i += 64) {
-            const uint32_t w = fidx[i];
-            uint64_t t = fixed_word(w) & ~bit_at(xv, w);
-            if (vq) t &= Eh[(uint64_t)(xv - rlo) * W + w];
-            if (vs) t &= gt_mask(xv, w);
-            cnt += (uint64_t)__popcll(t);
-            if (pl.emit && t) emit_word(out, pl, lc, u, A, t, w, xget, hs, hx);
-        }
-    };
-
     // DFS below level 2 with C_2 in my[0, W): x_2 runs over C_2 n [pos0, hi2).
     auto dfs3 = [&](uint32_t pos0, uint32_t hi2) {
         int l = 3;
         uint32_t pos = pos0;
-        if (lc == 3) build_fixed();  // fixed part depends on x_1 only
         for (;;) {
             if (++spins == kClockEvery) {
                 spins = 0;
@@ -803,8 +732,15 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             if (lane == 0) sX[wv][l - 1] = xi;
             __builtin_amdgcn_wave_barrier();
             const uint32_t lo = word_lo(l);
-            if (l == lc) {
-                count_last(lo);
+            if (l == L - 1) {
+                for (uint32_t rr = lo >> 6; rr < R; rr++) {
+                    const uint32_t w = rr * 64 + lane;
+                    if (w >= W) break;
+                    if (w < lo) continue;
+                    const uint64_t t = level_word(l, w);
+                    cnt += (uint64_t)__popcll(t);
+                    if (pl.emit && t) emit_word(out, pl, l, u, A, t, w, xget, hs, hx);
+                }
                 pos = xi + 1u;
             } else {
                 for (uint32_t rr = lo >> 6; rr < R; rr++) {
@@ -815,7 +751,6 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 __builtin_amdgcn_wave_barrier();
                 ++l;
                 pos = 0;
-                if (l == lc) build_fixed();  // x_1..x_{L-3} are now fixed
             }
         }
     };

@@ -85,12 +85,11 @@ struct HubQueue {
     uint32_t chunk;                 // candidates per record
 };
 
-// Words (u64) of one hub-DFS wave's buffer: nlev candidate-stack rows, the
-// bitmap row of x_1, and the compacted word indices (u32) of the last level's
-// fixed part.
+// Words (u64) of one hub-DFS wave's buffer: nlev candidate-stack rows and the
+// bitmap row of x_1.
 __host__ __device__ inline uint64_t hub_wave_words(int nlev, uint32_t wmax)
 {
-    return (uint64_t)(nlev + 1) * wmax + (wmax + 1) / 2;
+    return (uint64_t)(nlev + 1) * wmax;
 }
 
 struct KOut {
