@@ -13,9 +13,9 @@
 // others run are started right away and share the GPU with them.  A query
 // can carry a deadline after which its kernels stop (the result is then
 // marked stopped).  With a budget, each new plan first runs as a pilot on a
-// stratified 1/stride sample of its root queue (deadline 2 x budget/stride,
+// stratified 1/stride sample of its root queue (deadline 4 x budget/stride,
 // on the lane the query was admitted to); the pilot's device time times the
-// stride predicts the full run, and plans predicted above twice the budget
+// stride predicts the full run, and plans predicted above 4 x the budget
 // are refused without running.  Predictions are cached per plan.
 
 #include <hip/hip_runtime.h>
@@ -39,7 +39,7 @@ using Clock = std::chrono::steady_clock;
 // pilot sharing the GPU with running queries make predictions noisy; refusing
 // a query that would finish costs a completion, admitting one that would not
 // costs at most its deadline.
-constexpr double kPilotSlack = 2.0;
+constexpr double kPilotSlack = 4.0;
 
 static double ms_between(Clock::time_point a, Clock::time_point b)
 {

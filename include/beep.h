@@ -233,8 +233,8 @@ typedef struct {
     uint64_t arena_cap;     /* hub-arena bytes per query; 0 = pool / (2 * lanes) */
     double budget_ms;       /* > 0: predict each new plan's device time with a pilot run
                                on a 1/pilot_stride sample of its root queue (deadline
-                               2 budget_ms / pilot_stride) and refuse, with BEEP_ERANGE,
-                               plans predicted above 2 budget_ms */
+                               4 budget_ms / pilot_stride) and refuse, with BEEP_ERANGE,
+                               plans predicted above 4 budget_ms */
     uint32_t pilot_stride;  /* sample stride of the pilot (0 = 32) */
 } beep_server_opts;
 

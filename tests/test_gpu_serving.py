@@ -175,7 +175,7 @@ def test_server_prediction_refuses_heavy_plans(c2_graph):
     srv.submit(light, emit=B.EMIT_HASH)
     r2 = srv.poll(timeout_ms=120000)
     ref = alone(c2_graph, "triangle", B.SGC)
-    assert r2["status"] == 0 and 0.0 <= r2["predicted_ms"] <= 50.0
+    assert r2["status"] == 0 and 0.0 <= r2["predicted_ms"] <= 4 * 50.0
     assert (r2["count"], r2["hash_sum"], r2["hash_xor"]) == (ref["count"], ref["hash_sum"], ref["hash_xor"])
     pilots = srv.stats()["pilots"]
     srv.submit(heavy)                  # refused from the cached prediction, no new pilot
