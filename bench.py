@@ -542,12 +542,15 @@ def run_large(args):
         print(f"[bench] {json.dumps(rec)}", file=sys.stderr, flush=True)
     batched = None
     if len(ran) > 1:
-        b = B.Batch(ran)
-        res, t_first = b.run()
-        res, t = b.run()
-        batched = {"queries": [p.plan.k for p in ran], "ms": t, "first_launch_ms": t_first,
-                   "plain_ms": sum(r["t_enum_ms"] for r in rows if "t_enum_ms" in r and not r["stopped"]),
-                   "counts": [r["count"] for r in res]}
+        try:
+            b = B.Batch(ran)
+            res, t_first = b.run()
+            res, t = b.run()
+            batched = {"queries": [p.plan.k for p in ran], "ms": t, "first_launch_ms": t_first,
+                       "plain_ms": sum(r["t_enum_ms"] for r in rows if "t_enum_ms" in r and not r["stopped"]),
+                       "counts": [r["count"] for r in res]}
+        except B.BeepError as e:
+            batched = {"error": str(e)}
         print(f"[bench] batched {batched}", file=sys.stderr, flush=True)
     done = [r for r in rows if "t_enum_ms" in r and not r["stopped"]]
     tot_ms = sum(r["t_enum_ms"] for r in done)
