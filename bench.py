@@ -378,7 +378,7 @@ def run_serve(args):
             if r is not None:
                 r.pop("matches", None)
                 out[r["ticket"]] = r
-                if len(out) % 20 == 0:
+                if len(out) % 5 == 0:
                     print(f"[bench] {tag} {len(out)}/{n}", file=sys.stderr, flush=True)
         return out
 
@@ -396,8 +396,8 @@ def run_serve(args):
         r = plain_srv.poll()
         r.pop("matches", None)
         plain.append(r)
-        if i % 20 == 0:
-            print(f"[bench] plain {i}/{N}", file=sys.stderr, flush=True)
+        print(f"[bench] plain {i + 1}/{N}: status {r['status']} {r['latency_ms']:.1f} ms", file=sys.stderr,
+              flush=True)
     plain_wall = time.time() - t0
     plain_srv.close()
     svc = [r["latency_ms"] for r in plain]
