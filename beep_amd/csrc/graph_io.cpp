@@ -296,13 +296,18 @@ static std::unique_ptr<Graph> csr_from_pairs(const std::vector<uint32_t> &a,
     }, 1 << 18);
     progress("csr: %llu entries scattered", (unsigned long long)off[nmax]);
     std::vector<uint64_t> len(nmax, 0);
-    parallel_for(nmax, [&](size_t lo, size_t hi) {
-        for (size_t v = lo; v < hi; v++) {
-            auto bgn = tmp.begin() + (ptrdiff_t)off[v], end = tmp.begin() + (ptrdiff_t)off[v + 1];
-            std::sort(bgn, end);
-            len[v] = (uint64_t)(std::unique(bgn, end) - bgn);
-        }
-    }, 1 << 12);
+    const uint32_t nslices = off[nmax] > (1ull << 30) ? 16 : 1;
+    for (uint32_t sl = 0; sl < nslices; sl++) {
+        const size_t s0 = (size_t)nmax * sl / nslices, s1 = (size_t)nmax * (sl + 1) / nslices;
+        parallel_for(s1 - s0, [&](size_t lo, size_t hi) {
+            for (size_t v = s0 + lo; v < s0 + hi; v++) {
+                auto bgn = tmp.begin() + (ptrdiff_t)off[v], end = tmp.begin() + (ptrdiff_t)off[v + 1];
+                std::sort(bgn, end);
+                len[v] = (uint64_t)(std::unique(bgn, end) - bgn);
+            }
+        }, 1 << 12);
+        if (nslices > 1) progress("csr: rows %zu..%zu sorted", s0, s1);
+    }
     progress("csr: rows sorted and deduplicated");
     g->rowptr.assign((size_t)nmax + 1, 0);
     for (uint32_t v = 0; v < nmax; v++) g->rowptr[v + 1] = g->rowptr[v] + len[v];

@@ -119,8 +119,12 @@ std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc_flag
     V.nlow.assign(n, 0);
     V.hlen.assign(n, 0);
     const bool asc = asc_flag != 0;
-    parallel_for(n, [&](size_t b, size_t e) {
-        for (size_t u = b; u < e; u++) {
+    // in slices, so very large graphs report progress between them
+    const uint32_t nslices = V.m > (1ull << 30) ? 16 : 1;
+    for (uint32_t sl = 0; sl < nslices; sl++) {
+    const size_t s0 = (size_t)n * sl / nslices, s1 = (size_t)n * (sl + 1) / nslices;
+    parallel_for(s1 - s0, [&](size_t b, size_t e) {
+        for (size_t u = s0 + b; u < s0 + e; u++) {
             const uint64_t rb = rp[u], re = rp[u + 1];
             uint32_t du = deg[u], lo = 0;
             for (uint64_t x = rb; x < re; x++) {
@@ -154,6 +158,8 @@ std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc_flag
                 });
         }
     }, 4096);
+    if (nslices > 1) progress("orientation: rows %zu..%zu done", s0, s1);
+    }
     uint32_t maxd = 0;
     for (uint32_t v = 0; v < n; v++) maxd = std::max(maxd, deg[v]);
     V.max_degree = maxd;
