@@ -22,7 +22,7 @@ def golden(name):
 
 def cases():
     out = []
-    for name in ("rmat22cut", "chunglu1m"):
+    for name in ("rmat16", "rmat22cut", "chunglu1m"):
         path = os.path.join(HERE, "golden", f"{name}_counts.json")
         if not os.path.exists(path):
             continue
