@@ -14,6 +14,7 @@ from . import _ffi
 from ._ffi import BeepError, check, lib
 
 SGM, SGC = 0, 1
+BUILD_AUTO, BUILD_SCAN, BUILD_SEARCH = 0, 1, 2
 EMIT_NONE, EMIT_HASH, EMIT_BUFFER = 0, 1, 2
 
 __all__ = ["Graph", "Plan", "Prepared", "Batch", "Server", "SG_Match", "SGM", "SGC", "BeepError",
@@ -175,11 +176,11 @@ class Prepared:
         check(lib().beep_prep_upload(self._h, device))
 
     def count(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0,
-              deadline_ms=0.0, profile=False):
+              deadline_ms=0.0, profile=False, build_mode=0):
         """Runs the enumeration.  With a deadline the result may be partial:
         then ``stopped`` is 1 (the C call's BEEP_ETIMEDOUT is not raised).
         With ``profile`` the balance figures are filled in."""
-        o = _opts(device, emit, lanes, donate, donate_min, deadline_ms, profile)
+        o = _opts(device, emit, lanes, donate, donate_min, deadline_ms, profile, build_mode)
         buf = None
         if emit == EMIT_BUFFER and match_cap:
             buf = np.zeros(match_cap * self.plan.width, dtype=np.uint32)
@@ -201,7 +202,8 @@ class Prepared:
             self._h = C.c_void_p()
 
 
-def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline_ms=0.0, profile=False):
+def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline_ms=0.0, profile=False,
+          build_mode=0):
     o = _ffi.Opts()
     lib().beep_opts_default(C.byref(o))
     o.device, o.emit, o.lanes = device, emit, lanes
@@ -210,6 +212,7 @@ def _opts(device=0, emit=EMIT_NONE, lanes=0, donate=True, donate_min=0, deadline
         o.donate_min = donate_min
     o.deadline_ms = float(deadline_ms)
     o.profile = 1 if profile else 0
+    o.build_mode = build_mode
     return o
 
 
@@ -218,7 +221,7 @@ def _result_dict(r):
                 emitted=r.n_emitted, written=r.n_written, t_enum_ms=r.t_enum_ms,
                 t_hub_ms=r.t_hub_ms, t_lds_ms=r.t_lds_ms, donations=r.donations,
                 stopped=r.stopped, balance_lds=r.balance_lds, balance_hub=r.balance_hub,
-                donated_frac=r.donated_frac)
+                donated_frac=r.donated_frac, lds_build_frac=r.lds_build_frac)
 
 
 class Batch:

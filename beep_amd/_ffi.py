@@ -37,7 +37,7 @@ class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("emit", C.c_int), ("match_cap", C.c_uint64),
                 ("match_buf", u32p), ("lanes", C.c_int), ("donate", C.c_int),
                 ("donate_min", C.c_uint32), ("timing", C.c_int), ("deadline_ms", C.c_double),
-                ("profile", C.c_int)]
+                ("profile", C.c_int), ("build_mode", C.c_int)]
 
 
 class Result(C.Structure):
@@ -45,7 +45,7 @@ class Result(C.Structure):
                 ("n_emitted", C.c_uint64), ("n_written", C.c_uint64), ("t_enum_ms", C.c_double),
                 ("t_hub_ms", C.c_double), ("t_lds_ms", C.c_double), ("donations", C.c_uint64),
                 ("stopped", C.c_uint64), ("balance_lds", C.c_double), ("balance_hub", C.c_double),
-                ("donated_frac", C.c_double)]
+                ("donated_frac", C.c_double), ("lds_build_frac", C.c_double)]
 
 
 class ServerOpts(C.Structure):

@@ -300,6 +300,8 @@ int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_resul
         if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
             throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
         if (!(opt.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
+        if (opt.build_mode < BEEP_BUILD_AUTO || opt.build_mode > BEEP_BUILD_SEARCH)
+            throw Error(BEEP_EINVAL, "bad build mode");
         memset(out, 0, sizeof(*out));
         beep::count_on_device(*pp->pp, *p->p, opt, *out);
         if (out->stopped) throw Error(BEEP_ETIMEDOUT, "deadline passed: the counts are partial");
@@ -377,6 +379,8 @@ int beep_server_submit(beep_server *s, const beep_plan *p, const beep_opts *o, u
         if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
             throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
         if (!(opt.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
+        if (opt.build_mode < BEEP_BUILD_AUTO || opt.build_mode > BEEP_BUILD_SEARCH)
+            throw Error(BEEP_EINVAL, "bad build mode");
         *ticket = beep::server_submit(*s->s, *p->p, opt);
     });
 }

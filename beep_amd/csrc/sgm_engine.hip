@@ -131,6 +131,8 @@ static void upload_variant(Variant &V, int device)
         if (V.dev->device != device) throw Error(BEEP_EINVAL, "the prepared graph is resident on another device");
         return;
     }
+    if (V.m >= (1ull << kRangeShift) || V.max_degree >= (1u << (64 - kRangeShift)))
+        throw Error(BEEP_ERANGE, "graph too large for the staged row ranges (2^40 entries, degree 2^24)");
     auto *dg = new DevGraph();
     try {
         dg->device = device;
@@ -330,13 +332,14 @@ ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, in
     return n;
 }
 
-static KPlan make_kplan(const Plan &p, int emit)
+static KPlan make_kplan(const Plan &p, int emit, int build_mode)
 {
     KPlan k{};
     k.k = p.k;
     k.L = p.L;
     k.skip_split = p.skip_split;
     k.emit = emit;
+    k.bin_ratio = build_mode == BEEP_BUILD_SCAN ? ~0u : build_mode == BEEP_BUILD_SEARCH ? 0u : 32u;
     for (int l = 0; l < kMaxLevels; l++) {
         k.qmask[l] = p.qmask[l];
         k.smask[l] = p.smask[l];
@@ -356,8 +359,8 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     const bool buf = o.emit == BEEP_EMIT_BUFFER && o.match_cap && s.dbuf;
     c.out = KOut{s.ctr + kCtrAcc, buf ? s.dbuf : nullptr, buf ? o.match_cap : 0, s.ctr + kCtrDeadline,
                  s.ctr + kCtrStop, s.prof};
-    c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0);
-    c.kg = KGraph{V.dev->ocol, V.dev->vinfo};
+    c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0, o.build_mode);
+    c.kg = KGraph{V.dev->ocol, V.dev->vinfo, V.asc};
     c.nlev = p.L > 3 ? p.L - 3 : 1;
     c.hub_grid = hub_grid_of(R, c.nlev, cus);
     // donation ring (hub items with |C_2| > donate_min)
@@ -485,8 +488,10 @@ void fill_profile(const unsigned long long *prof, beep_result &res)
         return n && mx > 0 ? sum / n / mx : 0.0;
     };
     std::vector<double> lds(kProfSlots), hub(kProfSlots);
-    double own = 0, don = 0;
+    double own = 0, don = 0, lds_sum = 0, build = 0;
     for (int i = 0; i < kProfSlots; i++) {
+        lds_sum += (double)prof[i];
+        build += (double)prof[3 * kProfSlots + i];
         lds[i] = (double)prof[i];
         const double a = (double)prof[kProfSlots + i], b = (double)prof[2 * kProfSlots + i];
         hub[i] = a + b;
@@ -496,6 +501,7 @@ void fill_profile(const unsigned long long *prof, beep_result &res)
     res.balance_lds = balance(lds);
     res.balance_hub = balance(hub);
     res.donated_frac = (own + don) > 0 ? don / (own + don) : 0.0;
+    res.lds_build_frac = lds_sum > 0 ? build / lds_sum : 0.0;
 }
 
 // Sizes (grow-only) and carves the owned scratch region of a single query.

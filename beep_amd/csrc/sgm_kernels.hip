@@ -165,6 +165,25 @@ __device__ __forceinline__ void emit_word(const KOut &out, const KPlan &pl, int 
     }
 }
 
+// (deg a, a) comes before (deg b, b) in an oriented row
+__device__ __forceinline__ bool row_before(bool asc, uint32_t da, uint32_t a, uint32_t db, uint32_t b)
+{
+    if (da != db) return asc ? da < db : da > db;
+    return a < b;
+}
+
+// a has a larger key than v, i.e. lies in v's larger-key range when adjacent
+__device__ __forceinline__ bool larger_key(bool asc, uint32_t da, uint32_t a, uint32_t dv, uint32_t v)
+{
+    if (da != dv) return da > dv;
+    return asc ? a > v : a < v;
+}
+
+__device__ __forceinline__ bool use_search(uint32_t hlen, uint32_t d, uint32_t ratio)
+{
+    return (uint64_t)hlen > (uint64_t)d * ratio;
+}
+
 // ------------------------------------------------------------ LDS batches
 
 struct LdsLayout {
@@ -276,6 +295,7 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
     // bitmaps are built only when some level intersects with a bitmap row
     bool needE = false;
     for (int l = 2; l < L; l++) needE |= sMisc[M_QM + l] != 0u;
+    const uint64_t t_build = wall_clock64();
 
     if (tid < (int)nr) sRoot[tid] = recs[first + tid];
     __syncthreads();
@@ -313,8 +333,9 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             s = (s + 1) & (H - 1);
         }
         const VInfo vi = g.vinfo[vx];
-        sHBeg[j] = vi.hbeg;
-        sPre[j] = vi.hlen;
+        sHBeg[j] = vi.hbeg | ((uint64_t)vi.hlen << kRangeShift);
+        // work units: the range's entries, or the root's neighbours to look up
+        sPre[j] = use_search(vi.hlen, rr.d, pl.bin_ratio) ? rr.d : vi.hlen;
     }
     __syncthreads();
     if (needE) {
@@ -330,9 +351,31 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             const uint32_t j = lo;
             const RootRec rr = sRoot[sJRoot[j]];
             const uint32_t jl = j - rr.doff;
-            const uint32_t w = g.ocol[sHBeg[j] + (e - sPre[j])];
-            const uint32_t H = hash_slots_dev(rr.d);
             const uint32_t W = (rr.d + 63) >> 6;
+            const uint64_t hb = sHBeg[j] & ((1ull << kRangeShift) - 1ull);
+            const uint32_t hl = (uint32_t)(sHBeg[j] >> kRangeShift);
+            const uint32_t off = e - sPre[j];
+            if (use_search(hl, rr.d, pl.bin_ratio)) {
+                // neighbour i of the root: is A[i] in A[j]'s larger-key range?
+                const uint32_t i = off;
+                if (i == jl) continue;
+                const uint32_t a = sA[rr.doff + i], v = sA[j];
+                const uint32_t da = g.vinfo[a].deg, dv = g.vinfo[v].deg;
+                if (!larger_key(g.asc, da, a, dv, v)) continue;
+                uint32_t blo = 0, bhi = hl;  // first entry not before a
+                while (blo < bhi) {
+                    const uint32_t m = (blo + bhi) >> 1;
+                    const uint32_t x = g.ocol[hb + m];
+                    if (row_before(g.asc, g.vinfo[x].deg, x, da, a)) blo = m + 1; else bhi = m;
+                }
+                if (blo < hl && g.ocol[hb + blo] == a) {
+                    atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
+                    atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                }
+                continue;
+            }
+            const uint32_t w = g.ocol[hb + off];
+            const uint32_t H = hash_slots_dev(rr.d);
             uint32_t s = hash_slot(w, H);
             for (;;) {
                 const uint32_t key = sHKey[rr.hoff + s];
@@ -347,6 +390,8 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
         }
         __syncthreads();
     }
+
+    if (tid == 0) prof_add(out, 3, blockIdx.x, t_build);  // staging + bitmap build
 
     // DFS over (root, x_1) items
     uint32_t spins = 0;

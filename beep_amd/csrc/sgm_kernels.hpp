@@ -22,6 +22,10 @@ struct KPlan {
     int32_t L;           // k - LUNMAT
     int32_t skip_split;  // skip x_1 < split(u) (sgm_kernelsLD.cuh:60)
     int32_t emit;        // BEEP_EMIT_*
+    // LDS bitmap build: a row vertex whose larger-key range is longer than
+    // bin_ratio x the root's degree looks the root's neighbours up in that
+    // range by binary search instead of streaming it (0: always, ~0u: never)
+    uint32_t bin_ratio;
     uint32_t qmask[kMaxLevels];
     uint32_t smask[kMaxLevels];
 };
@@ -38,7 +42,11 @@ struct VInfo {
 struct KGraph {
     const uint32_t *ocol;    // degree-oriented rows
     const VInfo *vinfo;      // n records
+    int32_t asc;             // rows ascend by (degree, id); else degree descends, ids ascend
 };
+
+// Staged range of a row vertex: start (40 bits) and length (24 bits).
+constexpr int kRangeShift = 40;
 
 // Per-root record of an LDS batch, offsets relative to the batch.
 struct RootRec {
@@ -103,11 +111,12 @@ struct KOut {
     unsigned long long *stop;
     // Optional busy-time accumulators (wall_clock64 ticks), kProfPhases x
     // kProfSlots: phase 0 LDS workgroups, 1 hub waves on their own items, 2 hub
-    // waves on donated subtrees; slot = workgroup (LDS) or wave (hub) index.
+    // waves on donated subtrees, 3 the LDS workgroups' staging and bitmap
+    // build; slot = workgroup (LDS) or wave (hub) index.
     unsigned long long *prof;
 };
 
-constexpr int kProfSlots = 4096, kProfPhases = 3;
+constexpr int kProfSlots = 4096, kProfPhases = 4;
 
 // One prepared query of a multi-query launch.
 struct MultiQuery {

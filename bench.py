@@ -170,7 +170,8 @@ def main():
     for (q, _), (_, _, _, pp) in zip(queries, preps):
         r = pp.count(emit=B.EMIT_NONE, lanes=args.lanes, profile=True)
         balance[q] = {"lds": round(r["balance_lds"], 3), "hub": round(r["balance_hub"], 3),
-                      "donated_frac": round(r["donated_frac"], 3)}
+                      "donated_frac": round(r["donated_frac"], 3),
+                      "lds_build_frac": round(r["lds_build_frac"], 3)}
     K = args.steps
     dev_ms_step = sum(per_query_ms) / K
     matches = sum(counts)

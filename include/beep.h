@@ -175,7 +175,13 @@ typedef struct {
     double deadline_ms;     /* > 0: stop the enumeration this long after it starts on
                                the device; the result is then partial and marked stopped */
     int profile;            /* 1 = busy-time balance of workgroups / waves (beep_count) */
+    int build_mode;         /* BEEP_BUILD_*: how LDS bitmaps find adjacent row pairs */
 } beep_opts;
+
+#define BEEP_BUILD_AUTO 0   /* stream ranges, look up the root's neighbours when a
+                               range is over 32x the root's degree */
+#define BEEP_BUILD_SCAN 1   /* always stream the row vertices' larger-key ranges */
+#define BEEP_BUILD_SEARCH 2 /* always binary-search the root's neighbours */
 
 typedef struct {
     uint64_t count;         /* "------------- Counter = N" */
@@ -188,8 +194,9 @@ typedef struct {
     /* with opts.profile (the reference's TIMER build, sgm_kernelsLD_timer.cuh:7-64,
      * 339-349, as busy-time balance): mean / max busy time over the LDS
      * workgroups and over the hub-DFS waves (1 = perfectly even), and the
-     * share of hub-DFS busy time spent on donated subtrees */
-    double balance_lds, balance_hub, donated_frac;
+     * share of hub-DFS busy time spent on donated subtrees, and the share of
+     * LDS workgroup time spent staging rows and building bitmaps */
+    double balance_lds, balance_hub, donated_frac, lds_build_frac;
 } beep_result;
 
 void beep_opts_default(beep_opts *o);

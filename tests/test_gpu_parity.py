@@ -261,3 +261,34 @@ def test_profiled_run_counts_the_same(cl_graph, name, mode):
     assert 0.0 <= b["balance_hub"] <= 1.0 and 0.0 <= b["donated_frac"] <= 1.0
     if pp.arrays()["n_hub_roots"]:
         assert b["balance_hub"] > 0.0
+
+
+@pytest.mark.parametrize("build", [B.BUILD_SCAN, B.BUILD_SEARCH])
+@pytest.mark.parametrize("name,mode", [("triangle", B.SGC), ("diamond", B.SGM), ("4clique", B.SGM),
+                                       ("tailed_triangle", B.SGM), ("w5", B.SGM), ("5clique", B.SGC)])
+def test_bitmap_build_modes(cl_graph, name, mode, build):
+    """Streaming the larger-key ranges and binary-searching the root's
+    neighbours in them find the same adjacent pairs."""
+    g, og = cl_graph
+    say("build mode", build, name, mode)
+    s, d = P.directed(P.read_mtx_edges(name))
+    pp = B.Prepared(g, B.Plan.from_edges(s, d, mode))
+    pp.upload(0)
+    r = pp.count(emit=B.EMIT_HASH, build_mode=build)
+    o = O.Prepared(og, O.plan(s, d, mode)).count(want_hash=True)
+    assert (r["count"], r["hash_sum"], r["hash_xor"]) == (o["count"], o["hash_sum"], o["hash_xor"])
+
+
+@pytest.mark.parametrize("build", [B.BUILD_SCAN, B.BUILD_SEARCH])
+def test_bitmap_build_modes_small_graphs(build):
+    g = er_graph(70, 0.25, 11)
+    og = oracle_graph(g)
+    for name in ("triangle", "diamond", "4clique", "w5"):
+        for mode in (B.SGM, B.SGC):
+            say("build mode er70", build, name, mode)
+            s, d = P.directed(P.read_mtx_edges(name))
+            pp = B.Prepared(g, B.Plan.from_edges(s, d, mode))
+            pp.upload(0)
+            r = pp.count(emit=B.EMIT_HASH, build_mode=build)
+            o = O.Prepared(og, O.plan(s, d, mode)).count(want_hash=True)
+            assert (r["count"], r["hash_sum"], r["hash_xor"]) == (o["count"], o["hash_sum"], o["hash_xor"])
