@@ -64,11 +64,11 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t v, uint32_t H)
     return h & (H - 1u);
 }
 
+// smallest power of two >= max(2, 2d)
 __host__ __device__ __forceinline__ uint32_t hash_slots_dev(uint32_t d)
 {
-    uint32_t h = 2;
-    while (h < 2 * d) h <<= 1;
-    return h;
+    if (d <= 1) return 2;
+    return 1u << (32 - __builtin_clz(2 * d - 1));
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -341,51 +341,67 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
     if (needE) {
         block_excl_scan(sPre, sumd, sMisc + 28);
         const uint32_t total = sPre[sumd];
-        // bitmap: each adjacent pair (v, w), key(v) < key(w), is found from v
-        for (uint32_t e = tid; e < total; e += kBlock) {
-            uint32_t lo = 0, hi = sumd;  // largest j with sPre[j] <= e
+        // bitmap: each adjacent pair (v, w), key(v) < key(w), is found from v.
+        // Every thread takes kChunk consecutive work units: one search for the
+        // row of the first, then the rows advance with the units.
+        constexpr uint32_t kChunk = 8;
+        for (uint32_t e0 = tid * kChunk; e0 < total; e0 += kBlock * kChunk) {
+            uint32_t lo = 0, hi = sumd;  // largest j with sPre[j] <= e0
             while (hi - lo > 1) {
                 uint32_t mid = (lo + hi) >> 1;
-                if (sPre[mid] <= e) lo = mid; else hi = mid;
+                if (sPre[mid] <= e0) lo = mid; else hi = mid;
             }
-            const uint32_t j = lo;
-            const RootRec rr = sRoot[sJRoot[j]];
-            const uint32_t jl = j - rr.doff;
-            const uint32_t W = (rr.d + 63) >> 6;
-            const uint64_t hb = sHBeg[j] & ((1ull << kRangeShift) - 1ull);
-            const uint32_t hl = (uint32_t)(sHBeg[j] >> kRangeShift);
-            const uint32_t off = e - sPre[j];
-            if (use_search(hl, rr.d, pl.bin_ratio)) {
-                // neighbour i of the root: is A[i] in A[j]'s larger-key range?
-                const uint32_t i = off;
-                if (i == jl) continue;
-                const uint32_t a = sA[rr.doff + i], v = sA[j];
-                const uint32_t da = g.vinfo[a].deg, dv = g.vinfo[v].deg;
-                if (!larger_key(g.asc, da, a, dv, v)) continue;
-                uint32_t blo = 0, bhi = hl;  // first entry not before a
-                while (blo < bhi) {
-                    const uint32_t m = (blo + bhi) >> 1;
-                    const uint32_t x = g.ocol[hb + m];
-                    if (row_before(g.asc, g.vinfo[x].deg, x, da, a)) blo = m + 1; else bhi = m;
+            uint32_t j = lo;
+            uint32_t jn = sPre[j + 1];  // first unit of the next row (sPre[sumd] = total)
+            const uint32_t e1 = e0 + kChunk < total ? e0 + kChunk : total;
+            RootRec rr = sRoot[sJRoot[j]];
+            uint32_t j0 = sPre[j];
+            for (uint32_t e = e0; e < e1; e++) {
+                if (e >= jn) {
+                    do {
+                        j++;
+                        jn = sPre[j + 1];
+                    } while (e >= jn);
+                    rr = sRoot[sJRoot[j]];
+                    j0 = sPre[j];
                 }
-                if (blo < hl && g.ocol[hb + blo] == a) {
-                    atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
-                    atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                const uint32_t jl = j - rr.doff;
+                const uint32_t W = (rr.d + 63) >> 6;
+                const uint64_t hb = sHBeg[j] & ((1ull << kRangeShift) - 1ull);
+                const uint32_t hl = (uint32_t)(sHBeg[j] >> kRangeShift);
+                const uint32_t off = e - j0;
+                if (use_search(hl, rr.d, pl.bin_ratio)) {
+                    // neighbour i of the root: is A[i] in A[j]'s larger-key range?
+                    const uint32_t i = off;
+                    if (i == jl) continue;
+                    const uint32_t a = sA[rr.doff + i], v = sA[j];
+                    const uint32_t da = g.vinfo[a].deg, dv = g.vinfo[v].deg;
+                    if (!larger_key(g.asc, da, a, dv, v)) continue;
+                    uint32_t blo = 0, bhi = hl;  // first entry not before a
+                    while (blo < bhi) {
+                        const uint32_t m = (blo + bhi) >> 1;
+                        const uint32_t x = g.ocol[hb + m];
+                        if (row_before(g.asc, g.vinfo[x].deg, x, da, a)) blo = m + 1; else bhi = m;
+                    }
+                    if (blo < hl && g.ocol[hb + blo] == a) {
+                        atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
+                        atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                    }
+                    continue;
                 }
-                continue;
-            }
-            const uint32_t w = g.ocol[hb + off];
-            const uint32_t H = hash_slots_dev(rr.d);
-            uint32_t s = hash_slot(w, H);
-            for (;;) {
-                const uint32_t key = sHKey[rr.hoff + s];
-                if (key == 0u) break;
-                if (key == w + 1u) {
-                    const uint32_t i = sHVal[rr.hoff + s];
-                    atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
-                    atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                const uint32_t w = g.ocol[hb + off];
+                const uint32_t H = hash_slots_dev(rr.d);
+                uint32_t s = hash_slot(w, H);
+                for (;;) {
+                    const uint32_t key = sHKey[rr.hoff + s];
+                    if (key == 0u) break;
+                    if (key == w + 1u) {
+                        const uint32_t i = sHVal[rr.hoff + s];
+                        atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
+                        atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                    }
+                    s = (s + 1) & (H - 1);
                 }
-                s = (s + 1) & (H - 1);
             }
         }
         __syncthreads();
