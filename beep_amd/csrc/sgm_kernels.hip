@@ -354,23 +354,33 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             uint32_t j = lo;
             uint32_t jn = sPre[j + 1];  // first unit of the next row (sPre[sumd] = total)
             const uint32_t e1 = e0 + kChunk < total ? e0 + kChunk : total;
-            RootRec rr = sRoot[sJRoot[j]];
-            uint32_t j0 = sPre[j];
+            // per-row context, refreshed when the units cross into the next row
+            RootRec rr;
+            uint32_t j0, jl, W, H, hl;
+            uint64_t hb;
+            bool srch;
+            auto load_row = [&]() {
+                rr = sRoot[sJRoot[j]];
+                j0 = sPre[j];
+                jl = j - rr.doff;
+                W = (rr.d + 63) >> 6;
+                H = hash_slots_dev(rr.d);
+                const uint64_t hv = sHBeg[j];
+                hb = hv & ((1ull << kRangeShift) - 1ull);
+                hl = (uint32_t)(hv >> kRangeShift);
+                srch = use_search(hl, rr.d, pl.bin_ratio);
+            };
+            load_row();
             for (uint32_t e = e0; e < e1; e++) {
                 if (e >= jn) {
                     do {
                         j++;
                         jn = sPre[j + 1];
                     } while (e >= jn);
-                    rr = sRoot[sJRoot[j]];
-                    j0 = sPre[j];
+                    load_row();
                 }
-                const uint32_t jl = j - rr.doff;
-                const uint32_t W = (rr.d + 63) >> 6;
-                const uint64_t hb = sHBeg[j] & ((1ull << kRangeShift) - 1ull);
-                const uint32_t hl = (uint32_t)(sHBeg[j] >> kRangeShift);
                 const uint32_t off = e - j0;
-                if (use_search(hl, rr.d, pl.bin_ratio)) {
+                if (srch) {
                     // neighbour i of the root: is A[i] in A[j]'s larger-key range?
                     const uint32_t i = off;
                     if (i == jl) continue;
@@ -390,7 +400,6 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                     continue;
                 }
                 const uint32_t w = g.ocol[hb + off];
-                const uint32_t H = hash_slots_dev(rr.d);
                 uint32_t s = hash_slot(w, H);
                 for (;;) {
                     const uint32_t key = sHKey[rr.hoff + s];
@@ -635,31 +644,53 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int wb = wv * 64;
-        for (uint32_t e = lane; e < total; e += 64) {
-            int lo = 0, hi = 63;  // smallest k with pre[k] > e
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (sPre[wb + mid] > e) hi = mid; else lo = mid + 1;
+        // every lane takes kChunk consecutive elements: one search for the row
+        // of the first, then the rows advance with the elements
+        constexpr uint32_t kChunk = 8;
+        for (uint32_t e0 = lane * kChunk; e0 < total; e0 += 64 * kChunk) {
+            int k = 0, hi = 63;  // smallest k with pre[k] > e0
+            while (k < hi) {
+                const int mid = (k + hi) >> 1;
+                if (sPre[wb + mid] > e0) hi = mid; else k = mid + 1;
             }
-            const uint32_t before = lo ? sPre[wb + lo - 1] : 0u;
-            const uint32_t wv2 = g.ocol[sBeg[wb + lo] + (e - before)];
-            const HubRec h = hubs[sHub[wb + lo]];
-            const uint32_t jl = sJl[wb + lo];
-            const uint64_t W = (h.d + 63) >> 6;
-            uint32_t s = hash_slot(wv2, h.hslots);
-            for (;;) {
-                const uint32_t key = keys[h.hoff + s];
-                if (key == 0u) break;
-                if (key == wv2 + 1u) {
-                    const uint32_t i = vals[h.hoff + s];
-                    if (jl >= h.rlo)
-                        atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(jl - h.rlo) * W + (i >> 6)],
-                                 1ull << (i & 63));
-                    if (i >= h.rlo)
-                        atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(i - h.rlo) * W + (jl >> 6)],
-                                 1ull << (jl & 63));
+            uint32_t kend = sPre[wb + k];  // elements of row k: [kbeg, kend)
+            uint32_t kbeg = k ? sPre[wb + k - 1] : 0u;
+            HubRec h;
+            uint32_t jl = 0;
+            uint64_t W = 0, beg = 0;
+            auto load_row = [&]() {
+                h = hubs[sHub[wb + k]];
+                jl = sJl[wb + k];
+                W = (h.d + 63) >> 6;
+                beg = sBeg[wb + k];
+            };
+            load_row();
+            const uint32_t e1 = e0 + kChunk < total ? e0 + kChunk : total;
+            for (uint32_t e = e0; e < e1; e++) {
+                if (e >= kend) {
+                    do {
+                        k++;
+                        kbeg = kend;
+                        kend = sPre[wb + k];
+                    } while (e >= kend);
+                    load_row();
                 }
-                s = (s + 1) & (h.hslots - 1);
+                const uint32_t wv2 = g.ocol[beg + (e - kbeg)];
+                uint32_t s = hash_slot(wv2, h.hslots);
+                for (;;) {
+                    const uint32_t key = keys[h.hoff + s];
+                    if (key == 0u) break;
+                    if (key == wv2 + 1u) {
+                        const uint32_t i = vals[h.hoff + s];
+                        if (jl >= h.rlo)
+                            atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(jl - h.rlo) * W + (i >> 6)],
+                                     1ull << (i & 63));
+                        if (i >= h.rlo)
+                            atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(i - h.rlo) * W + (jl >> 6)],
+                                     1ull << (jl & 63));
+                    }
+                    s = (s + 1) & (h.hslots - 1);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
