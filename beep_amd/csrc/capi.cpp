@@ -106,6 +106,17 @@ static int guarded(F &&f)
     }
 }
 
+// Checks the fields of beep_opts every entry point shares.
+static void check_opts(const beep_opts &o)
+{
+    if (o.emit < BEEP_EMIT_NONE || o.emit > BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "bad emit mode");
+    if (o.emit == BEEP_EMIT_BUFFER && o.match_cap && !o.match_buf)
+        throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
+    if (!(o.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
+    if (o.build_mode < BEEP_BUILD_AUTO || o.build_mode > BEEP_BUILD_SEARCH) throw Error(BEEP_EINVAL, "bad build mode");
+    if (o.lanes < 0 || o.lanes > 64) throw Error(BEEP_EINVAL, "lanes must be 0 (automatic) or 1..64");
+}
+
 extern "C" {
 
 const char *beep_last_error(void) { return beep::g_last_error.c_str(); }
@@ -296,12 +307,7 @@ int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_resul
         beep_opts def;
         beep_opts_default(&def);
         const beep_opts &opt = o ? *o : def;
-        if (opt.emit < BEEP_EMIT_NONE || opt.emit > BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "bad emit mode");
-        if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
-            throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
-        if (!(opt.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
-        if (opt.build_mode < BEEP_BUILD_AUTO || opt.build_mode > BEEP_BUILD_SEARCH)
-            throw Error(BEEP_EINVAL, "bad build mode");
+        check_opts(opt);
         memset(out, 0, sizeof(*out));
         beep::count_on_device(*pp->pp, *p->p, opt, *out);
         if (out->stopped) throw Error(BEEP_ETIMEDOUT, "deadline passed: the counts are partial");
@@ -322,6 +328,7 @@ int beep_batch_create(beep_prep *const *preps, const beep_plan *const *plans, co
             lv[i] = plans[i]->p.get();
             if (opts) ov[i] = opts[i];
             else beep_opts_default(&ov[i]);
+            check_opts(ov[i]);
         }
         *out = new beep_batch{beep::batch_create(pv.data(), lv.data(), ov.data(), n)};
     });
@@ -375,12 +382,7 @@ int beep_server_submit(beep_server *s, const beep_plan *p, const beep_opts *o, u
         beep_opts def;
         beep_opts_default(&def);
         const beep_opts &opt = o ? *o : def;
-        if (opt.emit < BEEP_EMIT_NONE || opt.emit > BEEP_EMIT_BUFFER) throw Error(BEEP_EINVAL, "bad emit mode");
-        if (opt.emit == BEEP_EMIT_BUFFER && opt.match_cap && !opt.match_buf)
-            throw Error(BEEP_EINVAL, "match_buf is required with a match_cap");
-        if (!(opt.deadline_ms >= 0)) throw Error(BEEP_EINVAL, "bad deadline");
-        if (opt.build_mode < BEEP_BUILD_AUTO || opt.build_mode > BEEP_BUILD_SEARCH)
-            throw Error(BEEP_EINVAL, "bad build mode");
+        check_opts(opt);
         *ticket = beep::server_submit(*s->s, *p->p, opt);
     });
 }

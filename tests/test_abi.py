@@ -75,3 +75,27 @@ def test_cli_reports_missing_query(tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 1 and "unable to open" in r.stderr
     assert os.path.exists(P.mtx_path("triangle"))
+
+
+def test_bad_options_are_refused_before_any_device_call():
+    """Option checks run before anything touches a GPU, so they hold without one."""
+    import ctypes as C
+
+    import beep_amd as B
+    from beep_amd import _ffi
+    g = B.Graph.from_edges([0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1])
+    plan = B.Plan.from_undirected([(0, 1), (1, 2), (0, 2)], "sgc")
+    pp = B.Prepared(g, plan)
+    lib = _ffi.lib()
+    for field, value in (("build_mode", 7), ("deadline_ms", -1.0), ("lanes", 65), ("emit", 9)):
+        o = _ffi.Opts()
+        lib.beep_opts_default(C.byref(o))
+        setattr(o, field, value)
+        r = _ffi.Result()
+        assert lib.beep_count(pp._h, plan._h, C.byref(o), C.byref(r)) == -22, field
+        assert lib.beep_last_error()
+    o = _ffi.Opts()
+    lib.beep_opts_default(C.byref(o))
+    o.emit, o.match_cap = B.EMIT_BUFFER, 10
+    assert lib.beep_count(pp._h, plan._h, C.byref(o), C.byref(_ffi.Result())) == -22
+    assert b"match_buf" in lib.beep_last_error()
