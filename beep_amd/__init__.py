@@ -233,7 +233,8 @@ class Batch:
         n = len(self.prepared)
         tune = tune or [{}] * n
         self._opts = (_ffi.Opts * n)(*[_opts(device=device, emit=emit, lanes=t.get("lanes", 0),
-                                             donate_min=t.get("donate_min", 0)) for t in tune])
+                                             donate_min=t.get("donate_min", 0),
+                                             build_mode=t.get("build_mode", BUILD_AUTO)) for t in tune])
         preps = (C.c_void_p * n)(*[p._h.value for p in self.prepared])
         plans = (C.c_void_p * n)(*[p.plan._h.value for p in self.prepared])
         h = C.c_void_p()

@@ -284,7 +284,7 @@ def run_batched(args):
     t0 = time.time()
     for (q, _), pp, r in zip(queries, preps, plain):
         if r["t_enum_ms"] > args.tune_cap_ms:
-            tuned.append({"lanes": 0, "donate_min": 64, "ms": r["t_enum_ms"], "tuned": False})
+            tuned.append({"lanes": 0, "donate_min": 64, "build_mode": 0, "ms": r["t_enum_ms"], "tuned": False})
             continue
         best, ms, _ = T.tune(pp, reps=1)
         tuned.append(dict(best, ms=ms, tuned=True))
@@ -321,7 +321,8 @@ def run_batched(args):
         "batched_ms_first_launch": t_first,
         "speedup_vs_plain": plain_ms / batched_ms,
         "autotune": {"params": {q: t for (q, _), t in zip(queries, tuned)}, "seconds": round(t_tune, 2),
-                     "space": {"lanes": list(T.LANES), "donate_min": list(T.DONATE_MIN)}},
+                     "space": {"lanes": list(T.LANES), "donate_min": list(T.DONATE_MIN),
+                               "build_mode": list(T.BUILD)}},
         "parity": parity,
         "host_s": {"prepare": round(t_prep, 2)},
     }
