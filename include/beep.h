@@ -206,8 +206,8 @@ int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_resul
 /* ------------------------------------------------------------------------ */
 /* Batches.  No counterpart in the reference (one query per process there):  */
 /* several prepared queries over resident graphs run as one captured         */
-/* hipGraph -- per query its hub rounds on a branch of their own, and one    */
-/* persistent kernel over every query's LDS batches.  The graph is captured  */
+/* hipGraph -- one persistent kernel over every query's LDS batches, then    */
+/* per query its hub rounds on a branch of their own.  The graph is captured */
 /* on the first run and replayed afterwards.                                 */
 /* ------------------------------------------------------------------------ */
 
