@@ -292,3 +292,29 @@ def test_bitmap_build_modes_small_graphs(build):
             r = pp.count(emit=B.EMIT_HASH, build_mode=build)
             o = O.Prepared(og, O.plan(s, d, mode)).count(want_hash=True)
             assert (r["count"], r["hash_sum"], r["hash_xor"]) == (o["count"], o["hash_sum"], o["hash_xor"])
+
+
+def _clique(k):
+    return [(a, b) for a in range(k) for b in range(a + 1, k)]
+
+
+def _wheel(k):
+    return [(0, i) for i in range(1, k)] + [(i, i % (k - 1) + 1) for i in range(1, k)]
+
+
+_CEN10 = [(0, i) for i in range(1, 10)] + [(1, 2), (1, 5), (2, 6), (3, 4), (3, 8), (4, 9), (5, 7), (6, 9),
+                                           (7, 8), (2, 9), (1, 8)]
+
+
+@pytest.mark.parametrize("mode", [B.SGM, B.SGC])
+@pytest.mark.parametrize("name,edges,n,p", [("8clique", _clique(8), 40, 0.8), ("9clique", _clique(9), 40, 0.8),
+                                            ("10clique", _clique(10), 40, 0.8), ("w9", _wheel(9), 30, 0.5),
+                                            ("central10", _CEN10, 30, 0.5)])
+def test_largest_queries(name, edges, n, p, mode):
+    """Queries of 8-10 vertices (the reference's DEPTH = 10) against the oracle."""
+    g = er_graph(n, p, 7)
+    og = oracle_graph(g)
+    say("largest queries", name, mode)
+    r, o, _ = run_pair(g, og, edges, mode)
+    assert r["count"] == o["count"]
+    assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
