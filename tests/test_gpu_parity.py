@@ -318,3 +318,13 @@ def test_largest_queries(name, edges, n, p, mode):
     r, o, _ = run_pair(g, og, edges, mode)
     assert r["count"] == o["count"]
     assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
+
+
+@pytest.mark.parametrize("k", [6, 8, 10])
+def test_deep_cliques_on_hubs(cl_graph, k):
+    """Deep stacks on the hub path (a 10-clique keeps 7 candidate levels per wave)."""
+    g, og = cl_graph
+    say("deep clique hubs", k)
+    r, o, pp = run_pair(g, og, _clique(k), B.SGM)
+    assert r["count"] == o["count"]
+    assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
