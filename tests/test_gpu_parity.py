@@ -320,11 +320,11 @@ def test_largest_queries(name, edges, n, p, mode):
     assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
 
 
-@pytest.mark.parametrize("k", [6, 8, 10])
-def test_deep_cliques_on_hubs(cl_graph, k):
-    """Deep stacks on the hub path (a 10-clique keeps 7 candidate levels per wave)."""
+def test_deep_pattern_on_hubs(cl_graph):
+    """A 7-vertex wheel (4 candidate levels per wave) on the hub path."""
     g, og = cl_graph
-    say("deep clique hubs", k)
-    r, o, pp = run_pair(g, og, _clique(k), B.SGM)
+    say("deep pattern hubs w7")
+    r, o, pp = run_pair(g, og, _wheel(7), B.SGC)
+    assert pp.arrays()["n_hub_roots"] > 0
     assert r["count"] == o["count"]
     assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
