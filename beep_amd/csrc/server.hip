@@ -194,6 +194,7 @@ static ScratchNeeds max_needs(const ScratchNeeds &a, const ScratchNeeds &b)
     n.queue_cap = std::max(a.queue_cap, b.queue_cap);
     n.slot_cap = std::max(a.slot_cap, b.slot_cap);
     n.dbuf_words = std::max(a.dbuf_words, b.dbuf_words);
+    n.prof_words = std::max(a.prof_words, b.prof_words);
     return n;
 }
 
