@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--deadline-ms", type=float, default=1000.0, help="serve: per-query deadline")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--cold-sample", type=int, default=4, help="serve: queries timed the reference's way")
+    ap.add_argument("--max-k", type=int, default=7, help="serve: only patterns with at most this many vertices")
     ap.add_argument("--tune-cap-ms", type=float, default=1000.0,
                     help="batched: queries slower than this alone keep the default launch parameters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -353,7 +354,7 @@ def run_serve(args):
     t0 = time.time()
     g = W.make_graph(name)
     print(f"[bench] generated {name} in {time.time() - t0:.1f} s ({g.m // 2} edges)", file=sys.stderr, flush=True)
-    pats = P.central207()
+    pats = [p for p in P.central207() if p[0] <= args.max_k]
     rng = np.random.default_rng(args.seed)
     N = args.queries
     picks = rng.integers(0, len(pats), size=N)
@@ -474,9 +475,9 @@ def run_serve(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"{name}: {wl['kind']} n={g.n}, seed {wl['seed']}, {g.m // 2} undirected edges, "
-                               f"max degree {g.max_degree}; Poisson arrivals at {qps:.3f} q/s over the 207 "
-                               f"central-node patterns (3-7 vertices), 20% sgm with a {cap} tuple buffer, "
-                               f"per-query deadline {dl} ms",
+                               f"max degree {g.max_degree}; Poisson arrivals at {qps:.3f} q/s over the "
+                               f"{len(pats)} central-node patterns with 3-{args.max_k} vertices, 20% sgm with a "
+                               f"{cap} tuple buffer, per-query deadline {dl} ms",
                    "queries": N, "qps": qps, "deadline_ms": dl, "lanes": args.serve_lanes},
         "served": {"p50_ms": pct(lat_done, 50), "p99_ms": pct(lat_done, 99),
                    "p50_ms_all": pct(lat_all, 50), "p99_ms_all": pct(lat_all, 99),
