@@ -15,8 +15,9 @@
 // marked stopped).  With a budget, each new plan first runs as a pilot on a
 // stratified 1/stride sample of its root queue (deadline 4 x budget/stride,
 // on the lane the query was admitted to); the pilot's device time times the
-// stride predicts the full run, and plans predicted above 4 x the budget
-// are refused without running.  Predictions are cached per plan.
+// stride, over the average number of lanes busy while it ran, predicts the
+// full run alone, and plans predicted above 4 x the budget are refused
+// without running.  Predictions are cached per plan.
 
 #include <hip/hip_runtime.h>
 
@@ -105,6 +106,7 @@ struct Server {
         std::shared_ptr<Prep> pp;
         std::shared_ptr<Prep> pilot;  // sampled root queue while a pilot runs first
         bool piloting = false;
+        uint64_t busy_at_pilot = 1;  // lanes running when the pilot started (itself included)
         double predicted_ms = -1.0;
         std::string sig;
         ScratchNeeds need;
@@ -319,7 +321,15 @@ void Server::run()
                 float t = 0;
                 const bool ok = hipEventElapsedTime(&t, sl.lane.ev[0], sl.lane.ev[3]) == hipSuccess;
                 const bool stopped = sl.host_ctr[kCtrStop] != 0;
-                const double pred = (stopped || !ok) ? INFINITY : (double)t * stride;
+                // the pilot shared the GPU with the other running queries: its
+                // time over the average number of busy lanes estimates it alone
+                uint64_t busy_now = 1;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    busy_now = std::max<uint64_t>(st.running, 1);
+                }
+                const double share = 0.5 * (double)(sl.job.busy_at_pilot + busy_now);
+                const double pred = (stopped || !ok) ? INFINITY : (double)t * stride / share;
                 predicted[sl.job.sig] = pred;
                 sl.job.predicted_ms = pred;
                 sl.job.piloting = false;
@@ -396,6 +406,7 @@ void Server::run()
                 if (sl.job.piloting) {
                     std::lock_guard<std::mutex> lk(mu);
                     st.pilots++;
+                    sl.job.busy_at_pilot = std::max<uint64_t>(st.running, 1);
                 }
                 launch(sl);
             } catch (const Error &e) {
