@@ -173,9 +173,12 @@ struct Prep {
 // Fresh variant + root queue.
 std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p);
 // Root queue over an existing variant (which must match the plan's
-// min_qdeg and asc).  sample_stride > 1 keeps every sample_stride-th entry of
-// the work list (ordered by row length, so the sample is stratified by size).
+// min_qdeg and asc).  sample_stride > 1 keeps a stratified sample of the work
+// list (ordered by row length): every sample_stride-th LDS root and every
+// hub_sample_stride()-th hub root.
 std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &v, const Plan &p, uint32_t sample_stride = 1);
+// Stride at which a sampled root queue keeps hub roots (finer: hubs are few and heavy).
+inline uint32_t hub_sample_stride(uint32_t sample_stride) { return sample_stride >= 8 ? sample_stride / 8 : 1; }
 
 // LDS budget of the batch kernel (words / slots); shared with the kernels.
 struct LdsBudget {

@@ -239,10 +239,21 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
         if (pp->reduced)
             std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return len_of(a) > len_of(b); });
         if (sample_stride > 1) {
-            // the middle of every stride: the heaviest root alone does not
-            // stand for the stride-1 roots after it
+            // Hub roots (rows too long for an LDS batch, a prefix of the
+            // length-sorted list) are kept every hub_sample_stride(), the rest
+            // every sample_stride; the middle entry of every stride, so the
+            // heaviest root alone does not stand for the ones after it.
+            auto fits_lds = [](uint32_t d) {
+                const uint64_t w = (d + 63) / 64;
+                return (uint64_t)d * w <= LdsBudget::kMaxEWords && d <= LdsBudget::kMaxSumDeg &&
+                       hash_slots_for(d) <= LdsBudget::kMaxHash;
+            };
+            size_t nh = 0;
+            while (nh < idx.size() && !fits_lds(len_of(idx[nh]))) nh++;
+            const uint32_t hs = hub_sample_stride(sample_stride);
             size_t o = 0;
-            for (size_t i = sample_stride / 2; i < idx.size(); i += sample_stride) idx[o++] = idx[i];
+            for (size_t i = hs / 2; i < nh; i += hs) idx[o++] = idx[i];
+            for (size_t i = nh + sample_stride / 2; i < idx.size(); i += sample_stride) idx[o++] = idx[i];
             idx.resize(o);
         }
         pp->wu.resize(idx.size());

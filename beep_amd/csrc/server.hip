@@ -13,8 +13,9 @@
 // others run are started right away and share the GPU with them.  A query
 // can carry a deadline after which its kernels stop (the result is then
 // marked stopped).  With a budget, each new plan first runs as a pilot on a
-// stratified 1/stride sample of its root queue (deadline 4 x budget/stride,
-// on the lane the query was admitted to); the pilot's device time times the
+// stratified 1/stride sample of its root queue (hub roots sampled 8x finer,
+// deadline 4 x budget / the hub stride, on the lane the query was admitted to):
+// the larger of the pilot's hub and LDS phase times, each scaled by its
 // stride, over the average number of lanes busy while it ran, predicts the
 // full run alone, and plans predicted above 4 x the budget are refused
 // without running.  Predictions are cached per plan.
@@ -206,7 +207,8 @@ beep_opts Server::pilot_opts(const beep_opts &o) const
     po.emit = BEEP_EMIT_NONE;
     po.match_cap = 0;
     po.match_buf = nullptr;
-    po.deadline_ms = kPilotSlack * budget_ms / stride;
+    // long enough for the finer-sampled hub phase to reach kPilotSlack x budget
+    po.deadline_ms = kPilotSlack * budget_ms / hub_sample_stride(stride);
     return po;
 }
 
@@ -318,8 +320,11 @@ void Server::run()
             progress = true;
             if (q == hipSuccess && sl.job.piloting) {
                 // pilot done: predict, then refuse or run the query on the same lane
-                float t = 0;
-                const bool ok = hipEventElapsedTime(&t, sl.lane.ev[0], sl.lane.ev[3]) == hipSuccess;
+                float t_hub = 0, t_lds = 0;
+                const bool ok = hipEventElapsedTime(&t_hub, sl.lane.ev[0], sl.lane.ev[1]) == hipSuccess &&
+                                hipEventElapsedTime(&t_lds, sl.lane.ev[0], sl.lane.side_done[0]) == hipSuccess;
+                // hub and LDS phases run side by side; each scales by its own stride
+                const double t = std::max((double)t_hub * hub_sample_stride(stride) / stride, (double)t_lds);
                 const bool stopped = sl.host_ctr[kCtrStop] != 0;
                 // the pilot shared the GPU with the other running queries: its
                 // time over the average number of busy lanes estimates it alone

@@ -239,9 +239,9 @@ typedef struct {
     uint64_t pool_bytes;    /* scratch pool; 0 = a quarter of the free HBM */
     uint64_t arena_cap;     /* hub-arena bytes per query; 0 = pool / (2 * lanes) */
     double budget_ms;       /* > 0: predict each new plan's device time with a pilot run
-                               on a 1/pilot_stride sample of its root queue (deadline
-                               4 budget_ms / pilot_stride) and refuse, with BEEP_ERANGE,
-                               plans predicted above 4 budget_ms */
+                               on a stratified sample of its root queue (1/pilot_stride of
+                               the LDS roots, 8x more of the hub roots) and refuse, with
+                               BEEP_ERANGE, plans predicted above 4 budget_ms */
     uint32_t pilot_stride;  /* sample stride of the pilot (0 = 32) */
 } beep_server_opts;
 
