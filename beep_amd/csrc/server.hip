@@ -18,7 +18,10 @@
 // the larger of the pilot's hub and LDS phase times, each scaled by its
 // stride, over the average number of lanes busy while it ran, predicts the
 // full run alone, and plans predicted above 4 x the budget are refused
-// without running.  Predictions are cached per plan.
+// without running.  Predictions are cached per plan.  A query with a
+// prediction starts only when the predictions of the queries running beside
+// it and its own fit the budget together (or when nothing else runs), so
+// sharing the GPU does not push admitted queries past their deadline.
 
 #include <hip/hip_runtime.h>
 
@@ -109,6 +112,8 @@ struct Server {
         bool piloting = false;
         uint64_t busy_at_pilot = 1;  // lanes running when the pilot started (itself included)
         double predicted_ms = -1.0;
+        double charged_ms = 0;       // its share of running_pred while it runs
+        bool counted = false;        // counted in running_full / running_pred
         std::string sig;
         ScratchNeeds need;
         uint64_t off = 0, bytes = 0;
@@ -146,6 +151,14 @@ struct Server {
 
     std::deque<Job> pending;  // worker only
     std::map<std::string, double> predicted;  // worker only: plan -> predicted device ms
+    // worker only: queries (not pilots) running, and the sum of their predictions
+    int running_full = 0;
+    double running_pred = 0;
+    bool fits_time(const Job &j) const
+    {
+        return budget_ms <= 0 || j.predicted_ms < 0 || running_full == 0 ||
+               running_pred + j.predicted_ms <= budget_ms;
+    }
     std::thread worker;
 
     ~Server();
@@ -223,6 +236,10 @@ void Server::launch(Slot &sl)
     } else {
         sl.c = make_launch(*j.pp, j.plan, j.opts, sl.s, cus);
         enqueue_query(*j.pp, sl.c, sl.s, sl.lane, sl.host_ctr);
+        running_full++;
+        j.charged_ms = j.predicted_ms > 0 ? j.predicted_ms : 0;
+        running_pred += j.charged_ms;
+        j.counted = true;
     }
 }
 
@@ -282,6 +299,13 @@ void Server::finish(Slot &sl, int status, const std::string &err)
     d.latency_ms = ms_between(j.t_submit, now);
     alloc.release(j.off, j.bytes);
     sl.busy = false;
+    if (j.counted) {  // a query (not a pilot) ends: it no longer shares the GPU
+        running_full--;
+        running_pred -= j.charged_ms;
+        if (running_full == 0 || running_pred < 0) running_pred = 0;
+        j.counted = false;
+        j.charged_ms = 0;
+    }
     j.pp.reset();
     j.pilot.reset();
     std::lock_guard<std::mutex> lk(mu);
@@ -342,6 +366,18 @@ void Server::run()
                     finish(sl, BEEP_ERANGE, "predicted device time above the budget");
                     continue;
                 }
+                if (!fits_time(sl.job)) {
+                    // it would not finish in time beside the running queries:
+                    // give the lane and memory back and wait at the queue's head
+                    alloc.release(sl.job.off, sl.job.bytes);
+                    sl.busy = false;
+                    pending.push_front(std::move(sl.job));
+                    std::lock_guard<std::mutex> lk(mu);
+                    st.running--;
+                    st.pending++;
+                    st.pool_used = alloc.used();
+                    continue;
+                }
                 try {
                     launch(sl);
                 } catch (const Error &e) {
@@ -383,6 +419,7 @@ void Server::run()
                     continue;
                 }
             }
+            if (!j.piloting && !fits_time(j)) break;  // waits until running queries leave room
             Slot *free_slot = nullptr;
             for (auto &sp : slots)
                 if (!sp->busy) {
