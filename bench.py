@@ -496,7 +496,11 @@ def run_serve(args):
                      "speedup": (burst_done / burst_wall) / plain_done_rate if plain_done_rate else None},
         "predictor": {"refused_but_plain_completed": sum(1 for i in range(N) if by_idx[i]["status"] == B.ERANGE
                                                          and plain[i]["status"] == 0),
-                      "admitted_but_stopped": n_status(by_idx.values(), B.ETIMEDOUT)},
+                      "admitted_but_stopped": n_status(by_idx.values(), B.ETIMEDOUT),
+                      "misses": sorted({(stream[i][0], "sgm" if stream[i][1] else "sgc",
+                                         round(by_idx[i]["predicted_ms"], 1), round(plain[i]["t_enum_ms"], 1))
+                                        for i in range(N) if by_idx[i]["status"] == B.ERANGE
+                                        and plain[i]["status"] == 0})},
         "cold": {"sample": len(cold), "s_per_query": float(np.mean(cold)) if cold else None,
                  "what": "prepare (peel + orient + roots) + upload + count + free, per query"},
         "same_result_alone_and_in_mix": same,
