@@ -18,10 +18,12 @@
 // the larger of the pilot's hub and LDS phase times, each scaled by its
 // stride, over the average number of lanes busy while it ran, predicts the
 // full run alone, and plans predicted above 4 x the budget are refused
-// without running.  Predictions are cached per plan.  A query with a
-// prediction starts only when the predictions of the queries running beside
-// it and its own fit the budget together (or when nothing else runs), so
-// sharing the GPU does not push admitted queries past their deadline.
+// without running.  Predictions are cached per plan.  A pilot runs alone
+// (beside persistent kernels that hold the CUs its time says little), and a
+// query with a prediction starts only when the predictions of the queries
+// running beside it and its own fit the budget together (or when nothing
+// else runs), so sharing the GPU does not push admitted queries past their
+// deadline.
 
 #include <hip/hip_runtime.h>
 
@@ -122,6 +124,7 @@ struct Server {
         Lane lane;
         unsigned long long *host_ctr = nullptr;
         bool busy = false;
+        bool busy_pilot = false;  // its launch sequence is a pilot
         Job job;
         Scratch s;
         LaunchCfg c;
@@ -151,9 +154,11 @@ struct Server {
 
     std::deque<Job> pending;  // worker only
     std::map<std::string, double> predicted;  // worker only: plan -> predicted device ms
-    // worker only: queries (not pilots) running, and the sum of their predictions
+    // worker only: queries (not pilots) running, the sum of their predictions,
+    // and pilots running
     int running_full = 0;
     double running_pred = 0;
+    int running_pilots = 0;
     bool fits_time(const Job &j) const
     {
         return budget_ms <= 0 || j.predicted_ms < 0 || running_full == 0 ||
@@ -233,6 +238,7 @@ void Server::launch(Slot &sl)
         const beep_opts po = pilot_opts(j.opts);
         sl.c = make_launch(*j.pilot, j.plan, po, sl.s, cus);
         enqueue_query(*j.pilot, sl.c, sl.s, sl.lane, sl.host_ctr);
+        running_pilots++;
     } else {
         sl.c = make_launch(*j.pp, j.plan, j.opts, sl.s, cus);
         enqueue_query(*j.pp, sl.c, sl.s, sl.lane, sl.host_ctr);
@@ -299,6 +305,10 @@ void Server::finish(Slot &sl, int status, const std::string &err)
     d.latency_ms = ms_between(j.t_submit, now);
     alloc.release(j.off, j.bytes);
     sl.busy = false;
+    if (j.piloting && sl.busy_pilot) {  // a pilot ended with an error
+        running_pilots--;
+        sl.busy_pilot = false;
+    }
     if (j.counted) {  // a query (not a pilot) ends: it no longer shares the GPU
         running_full--;
         running_pred -= j.charged_ms;
@@ -344,6 +354,8 @@ void Server::run()
             progress = true;
             if (q == hipSuccess && sl.job.piloting) {
                 // pilot done: predict, then refuse or run the query on the same lane
+                running_pilots--;
+                sl.busy_pilot = false;
                 float t_hub = 0, t_lds = 0;
                 const bool ok = hipEventElapsedTime(&t_hub, sl.lane.ev[0], sl.lane.ev[1]) == hipSuccess &&
                                 hipEventElapsedTime(&t_lds, sl.lane.ev[0], sl.lane.side_done[0]) == hipSuccess;
@@ -419,7 +431,11 @@ void Server::run()
                     continue;
                 }
             }
-            if (!j.piloting && !fits_time(j)) break;  // waits until running queries leave room
+            // Pilots run alone: beside running queries (persistent kernels that
+            // hold the CUs) their time says little about the plan.  Queries
+            // wait while a pilot runs and until the running ones leave room.
+            if (j.piloting ? (running_full > 0 || running_pilots > 0) : (running_pilots > 0 || !fits_time(j)))
+                break;
             Slot *free_slot = nullptr;
             for (auto &sp : slots)
                 if (!sp->busy) {
@@ -451,6 +467,7 @@ void Server::run()
                     sl.job.busy_at_pilot = std::max<uint64_t>(st.running, 1);
                 }
                 launch(sl);
+                sl.busy_pilot = sl.job.piloting;
             } catch (const Error &e) {
                 (void)hipStreamSynchronize(sl.lane.main);
                 finish(sl, e.code, e.what());
