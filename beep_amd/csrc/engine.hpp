@@ -20,9 +20,9 @@ namespace beep {
 
 // Counter words of one launch sequence (u64): acc[4] | hub cursors[64] |
 // LDS cursors (32 x u32) | donation tail, head, slot tail, donated |
-// deadline | stop flag.
+// deadlines of the hub and the LDS kernels | stop flag.
 constexpr int kCtrAcc = 0, kCtrHub = 4, kCtrLds = 68, kCtrQTail = 84, kCtrQHead = 85, kCtrSlot = 86,
-              kCtrDonated = 87, kCtrDeadline = 88, kCtrStop = 89, kCtrWords = 90;
+              kCtrDonated = 87, kCtrDeadline = 88, kCtrStop = 90, kCtrWords = 91;
 
 // HBM copy of a variant: oriented rows and per-vertex records.
 struct DevGraph {
@@ -115,7 +115,8 @@ struct LaunchCfg {
     int nlev = 1;
     int lmin = 0;
     bool donate = false;
-    uint64_t deadline_ticks = 0;
+    uint64_t deadline_ticks = 0;      // hub kernels
+    uint64_t deadline_ticks_lds = 0;  // LDS kernels
 };
 
 int device_cus(int device);
@@ -125,7 +126,9 @@ uint64_t wall_clock_khz(int device);
 // rounds cut to arena_budget bytes (0: the default budget).
 void upload(Prep &pp, int device, int skip_split, uint64_t arena_budget = 0);
 ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, int cus);
-LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const Scratch &s, int cus);
+// lds_deadline_ms < 0: the LDS kernels share o.deadline_ms
+LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const Scratch &s, int cus,
+                      double lds_deadline_ms = -1.0);
 // clear counters + deadline, LDS classes on the side streams and hub rounds on
 // main, join, counters -> host_ctr; records lane.ev[0..3].
 void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane,

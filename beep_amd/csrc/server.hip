@@ -13,11 +13,11 @@
 // others run are started right away and share the GPU with them.  A query
 // can carry a deadline after which its kernels stop (the result is then
 // marked stopped).  With a budget, each new plan first runs as a pilot on a
-// stratified 1/stride sample of its root queue (hub roots sampled 8x finer,
-// deadline 4 x budget / the hub stride, on the lane the query was admitted to):
+// stratified 1/stride sample of its root queue (hub roots sampled 8x finer;
+// deadlines 1.5 x budget / the stride of each phase):
 // the larger of the pilot's hub and LDS phase times, each scaled by its
 // stride, over the average number of lanes busy while it ran, predicts the
-// full run alone, and plans predicted above 4 x the budget are refused
+// full run alone, and plans predicted above 1.5 x the budget are refused
 // without running.  Predictions are cached per plan.  A pilot runs alone
 // (beside persistent kernels that hold the CUs its time says little), and a
 // query with a prediction starts only when the predictions of the queries
@@ -42,11 +42,10 @@ namespace beep {
 using Clock = std::chrono::steady_clock;
 
 // A plan is refused when its predicted device time exceeds kPilotSlack x the
-// budget (the pilot's deadline scales the same way).  Sampling error and the
-// pilot sharing the GPU with running queries make predictions noisy; refusing
-// a query that would finish costs a completion, admitting one that would not
-// costs at most its deadline.
-constexpr double kPilotSlack = 4.0;
+// budget (the pilot's deadlines scale the same way).  Sampling error makes
+// predictions noisy; refusing a query that would finish costs a completion,
+// admitting one that would not costs at most its deadline.
+constexpr double kPilotSlack = 1.5;
 
 static double ms_between(Clock::time_point a, Clock::time_point b)
 {
@@ -225,7 +224,8 @@ beep_opts Server::pilot_opts(const beep_opts &o) const
     po.emit = BEEP_EMIT_NONE;
     po.match_cap = 0;
     po.match_buf = nullptr;
-    // long enough for the finer-sampled hub phase to reach kPilotSlack x budget
+    // hub phase: long enough for the finer-sampled hubs to reach kPilotSlack x
+    // budget; the LDS phase has its own deadline (kPilotSlack x budget / stride)
     po.deadline_ms = kPilotSlack * budget_ms / hub_sample_stride(stride);
     return po;
 }
@@ -236,7 +236,7 @@ void Server::launch(Slot &sl)
     Job &j = sl.job;
     if (j.piloting) {
         const beep_opts po = pilot_opts(j.opts);
-        sl.c = make_launch(*j.pilot, j.plan, po, sl.s, cus);
+        sl.c = make_launch(*j.pilot, j.plan, po, sl.s, cus, kPilotSlack * budget_ms / stride);
         enqueue_query(*j.pilot, sl.c, sl.s, sl.lane, sl.host_ctr);
         running_pilots++;
     } else {

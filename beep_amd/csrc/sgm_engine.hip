@@ -25,7 +25,7 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
                               uint32_t b0, uint32_t b1, unsigned int *cursor, uint32_t ewords,
                               const KOut &out, int *grid_out);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
-hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks, hipStream_t st);
+hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
@@ -347,7 +347,8 @@ static KPlan make_kplan(const Plan &p, int emit, int build_mode)
     return k;
 }
 
-LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const Scratch &s, int cus)
+LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const Scratch &s, int cus,
+                      double lds_deadline_ms)
 {
     const Variant &V = *pp.var;
     if (p.min_qdeg != V.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != V.asc ||
@@ -380,8 +381,13 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     c.lmin = 0;
     if (o.lanes > 0)
         while ((1 << c.lmin) < o.lanes && c.lmin < 6) c.lmin++;
-    c.deadline_ticks = o.deadline_ms > 0 ? (uint64_t)(o.deadline_ms * (double)wall_clock_khz(pp.dev->device)) : 0;
-    if (o.deadline_ms > 0 && c.deadline_ticks == 0) c.deadline_ticks = 1;
+    auto ticks = [&](double ms) -> uint64_t {
+        if (!(ms > 0)) return 0;
+        const uint64_t t = (uint64_t)(ms * (double)wall_clock_khz(pp.dev->device));
+        return t ? t : 1;
+    };
+    c.deadline_ticks = ticks(o.deadline_ms);
+    c.deadline_ticks_lds = lds_deadline_ms < 0 ? c.deadline_ticks : ticks(lds_deadline_ms);
     return c;
 }
 
@@ -448,7 +454,7 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
     hipStream_t st = lane.main;
     HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, st));
     if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, st));
-    HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, st));
+    HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
     HIP_CHECK(hipEventRecord(lane.ev[0], st));
     // The LDS lane classes run on the side streams, concurrently with the hub
     // rounds on the main stream.
@@ -671,7 +677,8 @@ static void batch_capture(Batch &b)
         const Scratch &s = b.scr[i];
         HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, b.s0));
         if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, b.s0));
-        HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, b.cfgs[i].deadline_ticks, b.s0));
+        HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, b.cfgs[i].deadline_ticks, b.cfgs[i].deadline_ticks_lds,
+                                      b.s0));
     }
     HIP_CHECK(launch_fill_zero(b.cursor, 4, b.s0));
     // The persistent LDS kernel runs first, on the whole GPU; the hub branches

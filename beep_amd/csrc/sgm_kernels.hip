@@ -523,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
         v.sMisc[M_SM + tid] = pl.smask[tid];
     }
     uint64_t cnt = 0, hs = 0, hx = 0;
-    const uint64_t dl = *out.dl;
+    const uint64_t dl = out.dl[1];
     const uint64_t t_start = wall_clock64();
     for (;;) {
         if (tid == 0) {
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__r
         }
         __syncthreads();
         uint64_t cnt = 0, hs = 0, hx = 0;
-        const uint64_t dl = *q.out.dl;
+        const uint64_t dl = q.out.dl[1];
         const uint64_t t_start = wall_clock64();
         process_batch<0>(q.g, pl, v, q.recs, q.batch_first[w.batch], q.batch_count[w.batch], q.out, dl,
                          1 << w.lanes_log2, cnt, hs, hx);
@@ -1069,15 +1069,20 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st)
     return hipGetLastError();
 }
 
-// Writes the deadline of one launch sequence: now + ticks, or none.
-__global__ void arm_deadline_kernel(unsigned long long *dl, uint64_t ticks)
+// Writes the deadlines of one launch sequence (hub kernels, LDS kernels):
+// now + ticks, or none.
+__global__ void arm_deadline_kernel(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds)
 {
-    if (threadIdx.x == 0) *dl = ticks ? (unsigned long long)wall_clock64() + ticks : ~0ull;
+    if (threadIdx.x == 0) {
+        const unsigned long long now = (unsigned long long)wall_clock64();
+        dl[0] = ticks_hub ? now + ticks_hub : ~0ull;
+        dl[1] = ticks_lds ? now + ticks_lds : ~0ull;
+    }
 }
 
-hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks, hipStream_t st)
+hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st)
 {
-    hipLaunchKernelGGL(arm_deadline_kernel, dim3(1), dim3(64), 0, st, dl, ticks);
+    hipLaunchKernelGGL(arm_deadline_kernel, dim3(1), dim3(64), 0, st, dl, ticks_hub, ticks_lds);
     return hipGetLastError();
 }
 

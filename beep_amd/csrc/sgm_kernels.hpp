@@ -104,9 +104,10 @@ struct KOut {
     unsigned long long *acc;  // [0] count [1] hash_sum [2] hash_xor [3] written
     uint32_t *buf;            // device tuple buffer (cap * L words) or null
     uint64_t cap;
-    // Deadline in wall_clock64() ticks, written on the device by the launch
-    // sequence's first kernel (all ones: none).  Kernels stop claiming work and
-    // leave their DFS once it has passed, and set *stop.
+    // Deadlines in wall_clock64() ticks, written on the device by the launch
+    // sequence's first kernel (all ones: none): dl[0] for the hub kernels,
+    // dl[1] for the LDS kernels.  Kernels stop claiming work and leave their
+    // DFS once theirs has passed, and set *stop.
     const unsigned long long *dl;
     unsigned long long *stop;
     // Optional busy-time accumulators (wall_clock64 ticks), kProfPhases x

@@ -240,8 +240,8 @@ typedef struct {
     uint64_t arena_cap;     /* hub-arena bytes per query; 0 = pool / (2 * lanes) */
     double budget_ms;       /* > 0: predict each new plan's device time with a pilot run
                                on a stratified sample of its root queue (1/pilot_stride of
-                               the LDS roots, 8x more of the hub roots) and refuse, with
-                               BEEP_ERANGE, plans predicted above 4 budget_ms */
+                               the LDS roots, 8x more of the hub roots; it runs alone) and
+                               refuse, with BEEP_ERANGE, plans predicted above 1.5 budget_ms */
     uint32_t pilot_stride;  /* sample stride of the pilot (0 = 32) */
 } beep_server_opts;
 
