@@ -184,14 +184,10 @@ __device__ __forceinline__ bool use_search(uint32_t hlen, uint32_t d, uint32_t r
     return (uint64_t)hlen > (uint64_t)d * ratio;
 }
 
-// One bit of a root's 64-bit row signature: a probe whose bit is clear cannot
-// be in the row, and skips the hash table.
-__device__ __forceinline__ uint32_t sig_bit(uint32_t v) { return (v * 0x85EBCA6Bu) >> 26; }
-
 // ------------------------------------------------------------ LDS batches
 
 struct LdsLayout {
-    uint32_t oE, oHBeg, oStk, oSig, oA, oPre, oHKey, oRoot, oHVal, oX, oJRoot, oMisc, bytes;
+    uint32_t oE, oHBeg, oStk, oA, oPre, oHKey, oRoot, oHVal, oX, oJRoot, oMisc, bytes;
     uint32_t ewords, nlev;
 };
 
@@ -206,7 +202,6 @@ __host__ inline LdsLayout make_lds_layout(uint32_t ewords, uint32_t nlev)
     L.oE = o;       o += ewords * 8;
     L.oHBeg = o;    o += LdsBudget::kMaxSumDeg * 8;
     L.oStk = o;     o += nlev * kBlock * 8;
-    L.oSig = o;     o += LdsBudget::kMaxRoots * 8;
     L.oRoot = o;    o += LdsBudget::kMaxRoots * (uint32_t)sizeof(RootRec);
     L.oA = o;       o += LdsBudget::kMaxSumDeg * 4;
     L.oPre = o;     o += align_up((LdsBudget::kMaxSumDeg + 1) * 4, 8);
@@ -250,7 +245,7 @@ __device__ void block_excl_scan(uint32_t *a, uint32_t n, uint32_t *tmp4)
 
 // LDS carve-up of one workgroup.
 struct LdsView {
-    uint64_t *sE, *sHBeg, *sStk, *sSig;
+    uint64_t *sE, *sHBeg, *sStk;
     RootRec *sRoot;
     uint32_t *sA, *sPre, *sHKey, *sMisc;
     uint16_t *sHVal, *sX;
@@ -263,7 +258,6 @@ __device__ __forceinline__ LdsView lds_view(unsigned char *smem, const LdsLayout
     v.sE = (uint64_t *)(smem + lay.oE);
     v.sHBeg = (uint64_t *)(smem + lay.oHBeg);
     v.sStk = (uint64_t *)(smem + lay.oStk);
-    v.sSig = (uint64_t *)(smem + lay.oSig);
     v.sRoot = (RootRec *)(smem + lay.oRoot);
     v.sA = (uint32_t *)(smem + lay.oA);
     v.sPre = (uint32_t *)(smem + lay.oPre);
@@ -312,7 +306,6 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
     if (needE) {
         for (uint32_t i = tid; i < ew; i += kBlock) sE[i] = 0;
         for (uint32_t i = tid; i < hsl; i += kBlock) sHKey[i] = 0;
-        for (uint32_t i = tid; i < nr; i += kBlock) v.sSig[i] = 0;
     }
     if (tid == 0) sMisc[M_NEXT] = 0;
     __syncthreads();
@@ -339,7 +332,6 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             }
             s = (s + 1) & (H - 1);
         }
-        atomicOr((unsigned long long *)&v.sSig[lo], 1ull << sig_bit(vx));
         const VInfo vi = g.vinfo[vx];
         sHBeg[j] = vi.hbeg | ((uint64_t)vi.hlen << kRangeShift);
         // work units: the range's entries, or the root's neighbours to look up
@@ -365,10 +357,9 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             // per-row context, refreshed when the units cross into the next row
             RootRec rr;
             uint32_t j0, jl, W, H, hl;
-            uint64_t hb, sig;
+            uint64_t hb;
             bool srch;
             auto load_row = [&]() {
-                sig = v.sSig[sJRoot[j]];
                 rr = sRoot[sJRoot[j]];
                 j0 = sPre[j];
                 jl = j - rr.doff;
@@ -409,7 +400,6 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                     continue;
                 }
                 const uint32_t w = g.ocol[hb + off];
-                if (!((sig >> sig_bit(w)) & 1ull)) continue;  // not in the root's row
                 uint32_t s = hash_slot(w, H);
                 for (;;) {
                     const uint32_t key = sHKey[rr.hoff + s];
