@@ -67,7 +67,7 @@ struct ScratchNeeds {
 // Scratch of one launch sequence, carved out of one device region.
 struct Scratch {
     uint64_t *arena = nullptr;
-    uint32_t *keys = nullptr, *vals = nullptr;
+    uint64_t *kv = nullptr;  // hub hash slots: key (vertex + 1) | row index << 32
     uint64_t *stk = nullptr;
     HubChunk *queue = nullptr;
     uint64_t *slots = nullptr;

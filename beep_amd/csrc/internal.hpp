@@ -181,11 +181,19 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &v, const Plan &
 inline uint32_t hub_sample_stride(uint32_t sample_stride) { return sample_stride >= 8 ? sample_stride / 8 : 1; }
 
 // LDS budget of the batch kernel (words / slots); shared with the kernels.
+// A batch of lane class lg holds roots of at most 64 << lg neighbours, so
+// its bitmaps take at most kMaxSumDeg << lg words.  The budget is sized so
+// that three to five workgroups share a CU (their memory latency overlaps).
 struct LdsBudget {
     static constexpr uint32_t kMaxRoots = 128;     // roots per batch
-    static constexpr uint32_t kMaxSumDeg = 1024;   // sum of degrees per batch
+    static constexpr uint32_t kMaxSumDeg = 512;    // sum of degrees per batch
     static constexpr uint32_t kMaxEWords = 4096;   // bitmap words (u64)
-    static constexpr uint32_t kMaxHash = 2048;     // hash slots
+    static constexpr uint32_t kMaxHash = 1024;     // hash slots
+    static constexpr int kMaxClass = 6;
+    static constexpr uint32_t ewords(int lg)
+    {
+        return (kMaxSumDeg << (lg < 3 ? lg : 3)) < kMaxEWords ? (kMaxSumDeg << (lg < 3 ? lg : 3)) : kMaxEWords;
+    }
 };
 
 uint32_t hash_slots_for(uint32_t d);

@@ -310,7 +310,7 @@ void build_tiers(Prep &pp)
         while (i < nr && b.count < LdsBudget::kMaxRoots) {
             uint32_t d = deg(i);
             uint64_t w = (d + 63) / 64;
-            if (b.count && (sd + d > cap || sw + d * w > LdsBudget::kMaxEWords ||
+            if (b.count && (sd + d > cap || sw + d * w > LdsBudget::ewords((int)lg) ||
                             sh + hash_slots_for(d) > LdsBudget::kMaxHash || lg_of(d) != lg))
                 break;
             sd += d;

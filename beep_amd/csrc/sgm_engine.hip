@@ -22,15 +22,15 @@ namespace beep {
 
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
-                              uint32_t b0, uint32_t b1, unsigned int *cursor, uint32_t ewords,
-                              const KOut &out, int *grid_out);
+                              uint32_t b0, uint32_t b1, unsigned int *cursor, const KOut &out,
+                              int *grid_out);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
 hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals);
+                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint32_t *keys,
-                             const uint32_t *vals, uint64_t *E, const KOut &out);
+                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint64_t *kv, uint64_t *E,
+                             const KOut &out);
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
@@ -279,7 +279,7 @@ static uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
 
 uint64_t ScratchNeeds::bytes() const
 {
-    return align256(kCtrWords * 8) + align256(arena_words * 8) + 2 * align256(hash_slots * 4) +
+    return align256(kCtrWords * 8) + align256(arena_words * 8) + align256(hash_slots * 8) +
            align256(stk_words * 8) + align256(queue_cap * sizeof(HubChunk)) + align256(slot_cap * 8) +
            align256(dbuf_words * 4) + align256(prof_words * 8);
 }
@@ -294,14 +294,28 @@ void carve(Scratch &s, char *base, const ScratchNeeds &n)
     };
     s.ctr = (unsigned long long *)take(kCtrWords * 8);
     s.arena = n.arena_words ? (uint64_t *)take(n.arena_words * 8) : nullptr;
-    s.keys = n.hash_slots ? (uint32_t *)take(n.hash_slots * 4) : nullptr;
-    s.vals = n.hash_slots ? (uint32_t *)take(n.hash_slots * 4) : nullptr;
+    s.kv = n.hash_slots ? (uint64_t *)take(n.hash_slots * 8) : nullptr;
     s.stk = n.stk_words ? (uint64_t *)take(n.stk_words * 8) : nullptr;
     s.queue = n.queue_cap ? (HubChunk *)take(n.queue_cap * sizeof(HubChunk)) : nullptr;
     s.slots = n.slot_cap ? (uint64_t *)take(n.slot_cap * 8) : nullptr;
     s.dbuf = n.dbuf_words ? (uint32_t *)take(n.dbuf_words * 4) : nullptr;
     s.prof = n.prof_words ? (unsigned long long *)take(n.prof_words * 8) : nullptr;
     s.cap = n;
+}
+
+// Count-only plans whose last level does not intersect with x_{L-2}'s row sum
+// that level in closed form (KPlan::tail).
+static bool tail_summed(const Plan &p, int emit)
+{
+    return emit == BEEP_EMIT_NONE && p.L >= 4 && !((p.qmask[p.L - 1] >> (p.L - 2)) & 1u);
+}
+
+// Hub items donate level-2 subtrees; not when everything below x_1 is one
+// summed level (L = 4 with a summed tail): an item then costs one pass over
+// its d/64 words whatever |C_2| is.
+static bool hub_donates(const Plan &p, const beep_opts &o, const DevRoots &R)
+{
+    return o.donate && R.nhubs && p.L >= 4 && !(p.L == 4 && tail_summed(p, o.emit));
 }
 
 static int hub_grid_of(const DevRoots &R, int nlev, int cus)
@@ -323,7 +337,7 @@ ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, in
         n.stk_words = (uint64_t)hub_grid_of(R, nlev, cus) * (kBlock / 64) *
                       hub_wave_words(nlev, std::max<uint32_t>(R.wmax, 1));
     }
-    if (o.donate && R.nhubs && p.L >= 4) {
+    if (hub_donates(p, o, R)) {
         n.queue_cap = std::max<uint64_t>(1, std::min<uint64_t>(1ull << 22, R.don_records));
         n.slot_cap = std::max<uint64_t>(1, std::min<uint64_t>(1ull << 26, R.don_words));
     }
@@ -340,6 +354,7 @@ static KPlan make_kplan(const Plan &p, int emit, int build_mode)
     k.skip_split = p.skip_split;
     k.emit = emit;
     k.bin_ratio = build_mode == BEEP_BUILD_SCAN ? ~0u : build_mode == BEEP_BUILD_SEARCH ? 0u : 32u;
+    k.tail = tail_summed(p, emit) ? 1 : 0;
     for (int l = 0; l < kMaxLevels; l++) {
         k.qmask[l] = p.qmask[l];
         k.smask[l] = p.smask[l];
@@ -365,7 +380,7 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     c.nlev = p.L > 3 ? p.L - 3 : 1;
     c.hub_grid = hub_grid_of(R, c.nlev, cus);
     // donation ring (hub items with |C_2| > donate_min)
-    c.donate = o.donate && R.nhubs && p.L >= 4 && s.queue && s.slots;
+    c.donate = hub_donates(p, o, R) && s.queue && s.slots;
     if (c.donate) {
         c.hq.q = s.queue;
         c.hq.cap = s.cap.queue_cap;
@@ -405,12 +420,12 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
         const uint64_t rows = r.g1 - r.g0;
         if (pp.uses_bitmap) {
             HIP_CHECK(launch_fill_zero(s.arena, r.ewords * 8, st));
-            HIP_CHECK(launch_fill_zero(s.keys, r.hslots * 4, st));
+            HIP_CHECK(launch_fill_zero(s.kv, r.hslots * 8, st));
             int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)cus * 16);
-            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.keys, s.vals));
+            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv));
             int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
-            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.keys,
-                                        s.vals, s.arena, c.out));
+            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv,
+                                        s.arena, c.out));
         }
         unsigned long long *cur = s.ctr + kCtrHub + (ri % 64);
         if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, 8, st));
@@ -439,11 +454,10 @@ static void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratc
         int lg = std::max<int>(pp.batch_lanes_log2[b], c.lmin);
         uint32_t e = b + 1;
         while (e < R.nbatches && std::max<int>(pp.batch_lanes_log2[e], c.lmin) == lg) e++;
-        uint32_t ewords = std::min<uint32_t>(LdsBudget::kMaxEWords, LdsBudget::kMaxSumDeg << std::min(lg, 3));
         if (cls >= 32) throw Error(BEEP_ERANGE, "too many lane classes");
         unsigned int *cur = (unsigned int *)(s.ctr + kCtrLds) + cls;
         HIP_CHECK(launch_lds_batches(lg, pp.dev->cus, streams[cls % nst], c.kg, c.kp, R.recs, R.batch_first,
-                                     R.batch_count, b, e, cur, ewords, c.out, nullptr));
+                                     R.batch_count, b, e, cur, c.out, nullptr));
         b = e;
         cls++;
     }

@@ -57,6 +57,36 @@ __device__ __forceinline__ uint64_t bit_at(uint32_t x, uint32_t w)
     return (x >> 6) == w ? (1ull << (x & 63u)) : 0ull;
 }
 
+// bits < x of word w
+__device__ __forceinline__ uint64_t lt_mask(uint32_t x, uint32_t w)
+{
+    const uint32_t xw = x >> 6;
+    if (xw > w) return ~0ull;
+    if (xw < w) return 0ull;
+    return (1ull << (x & 63u)) - 1ull;
+}
+
+// Pairs (x, y) with x in S, y in F and y > x, inside one word; loops over the
+// sparser operand.
+__device__ __forceinline__ uint32_t pairs_in_word(uint64_t S, uint64_t F)
+{
+    uint32_t n = 0;
+    if (__popcll(S) <= __popcll(F)) {
+        while (S) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(S);
+            S &= S - 1;
+            n += (uint32_t)__popcll((F >> b) >> 1);
+        }
+    } else {
+        while (F) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(F);
+            F &= F - 1;
+            n += (uint32_t)__popcll(S & ((1ull << b) - 1ull));
+        }
+    }
+    return n;
+}
+
 __device__ __forceinline__ uint32_t hash_slot(uint32_t v, uint32_t H)
 {
     uint32_t h = v * 0x9E3779B1u;
@@ -193,23 +223,26 @@ struct LdsLayout {
 
 __host__ __device__ inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
-__host__ inline LdsLayout make_lds_layout(uint32_t ewords, uint32_t nlev)
+// Layout of lane class lg's batches (the widest class serves every batch);
+// nlev + 3 >= L positions of x are kept per thread.
+__host__ inline LdsLayout make_lds_layout(int lg, uint32_t nlev)
 {
+    const uint32_t sd = LdsBudget::kMaxSumDeg, hs = LdsBudget::kMaxHash, ew = LdsBudget::ewords(lg);
     LdsLayout L{};
     uint32_t o = 0;
-    L.ewords = ewords;
+    L.ewords = ew;
     L.nlev = nlev;
-    L.oE = o;       o += ewords * 8;
-    L.oHBeg = o;    o += LdsBudget::kMaxSumDeg * 8;
+    L.oE = o;       o += ew * 8;
+    L.oHBeg = o;    o += sd * 8;
     L.oStk = o;     o += nlev * kBlock * 8;
     L.oRoot = o;    o += LdsBudget::kMaxRoots * (uint32_t)sizeof(RootRec);
-    L.oA = o;       o += LdsBudget::kMaxSumDeg * 4;
-    L.oPre = o;     o += align_up((LdsBudget::kMaxSumDeg + 1) * 4, 8);
-    L.oHKey = o;    o += LdsBudget::kMaxHash * 4;
+    L.oA = o;       o += sd * 4;
+    L.oPre = o;     o += align_up((sd + 1) * 4, 8);
+    L.oHKey = o;    o += hs * 4;
     L.oMisc = o;    o += 32 * 4;
-    L.oHVal = o;    o += LdsBudget::kMaxHash * 2;
-    L.oX = o;       o += kMaxLevels * kBlock * 2;
-    L.oJRoot = o;   o += LdsBudget::kMaxSumDeg;
+    L.oHVal = o;    o += hs * 2;
+    L.oX = o;       o += (nlev + 3) * kBlock * 2;
+    L.oJRoot = o;   o += sd;
     L.bytes = align_up(o, 16);
     return L;
 }
@@ -310,18 +343,43 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
     if (tid == 0) sMisc[M_NEXT] = 0;
     __syncthreads();
 
-    // rows, hash tables and "higher" ranges of every row vertex
-    for (uint32_t j = tid; j < sumd; j += kBlock) {
-        uint32_t lo = 0, hi = nr;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (sRoot[mid].doff <= j) lo = mid; else hi = mid;
+    // rows, hash tables and "higher" ranges of every row vertex.  A thread
+    // holds at most kRowsPer rows; their row entries, then their vertex
+    // records, are loaded together.
+    constexpr int kRowsPer = (int)(LdsBudget::kMaxSumDeg / kBlock);
+    static_assert(LdsBudget::kMaxSumDeg % kBlock == 0, "rows per thread");
+    uint32_t rvx[kRowsPer], rlo[kRowsPer];
+#pragma unroll
+    for (int q = 0; q < kRowsPer; q++) {
+        const uint32_t j = (uint32_t)tid + (uint32_t)q * kBlock;
+        rvx[q] = 0;
+        rlo[q] = 0;
+        if (j < sumd) {
+            uint32_t lo = 0, hi = nr;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (sRoot[mid].doff <= j) lo = mid; else hi = mid;
+            }
+            rlo[q] = lo;
+            const RootRec rr = sRoot[lo];
+            rvx[q] = g.ocol[rr.rbeg + (j - rr.doff)];
         }
-        const RootRec rr = sRoot[lo];
+    }
+    VInfo rvi[kRowsPer];
+    if (needE) {
+#pragma unroll
+        for (int q = 0; q < kRowsPer; q++)
+            if ((uint32_t)tid + (uint32_t)q * kBlock < sumd) rvi[q] = g.vinfo[rvx[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kRowsPer; q++) {
+        const uint32_t j = (uint32_t)tid + (uint32_t)q * kBlock;
+        if (j >= sumd) break;
+        const RootRec rr = sRoot[rlo[q]];
         const uint32_t jl = j - rr.doff;
-        const uint32_t vx = g.ocol[rr.rbeg + jl];
+        const uint32_t vx = rvx[q];
         sA[j] = vx;
-        sJRoot[j] = (uint8_t)lo;
+        sJRoot[j] = (uint8_t)rlo[q];
         if (!needE) continue;
         const uint32_t H = hash_slots_dev(rr.d);
         uint32_t s = hash_slot(vx, H);
@@ -332,7 +390,7 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             }
             s = (s + 1) & (H - 1);
         }
-        const VInfo vi = g.vinfo[vx];
+        const VInfo vi = rvi[q];
         sHBeg[j] = vi.hbeg | ((uint64_t)vi.hlen << kRangeShift);
         // work units: the range's entries, or the root's neighbours to look up
         sPre[j] = use_search(vi.hlen, rr.d, pl.bin_ratio) ? rr.d : vi.hlen;
@@ -343,7 +401,9 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
         const uint32_t total = sPre[sumd];
         // bitmap: each adjacent pair (v, w), key(v) < key(w), is found from v.
         // Every thread takes kChunk consecutive work units: one search for the
-        // row of the first, then the rows advance with the units.
+        // row of the first, then the rows advance with the units.  The units'
+        // entries are loaded in a first pass, back to back, and probed in a
+        // second, so a thread waits once for its kChunk loads.
         constexpr uint32_t kChunk = 8;
         for (uint32_t e0 = tid * kChunk; e0 < total; e0 += kBlock * kChunk) {
             uint32_t lo = 0, hi = sumd;  // largest j with sPre[j] <= e0
@@ -351,10 +411,10 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                 uint32_t mid = (lo + hi) >> 1;
                 if (sPre[mid] <= e0) lo = mid; else hi = mid;
             }
-            uint32_t j = lo;
-            uint32_t jn = sPre[j + 1];  // first unit of the next row (sPre[sumd] = total)
+            const uint32_t jfirst = lo;
             const uint32_t e1 = e0 + kChunk < total ? e0 + kChunk : total;
             // per-row context, refreshed when the units cross into the next row
+            uint32_t j, jn;  // row, first unit of the next row (sPre[sumd] = total)
             RootRec rr;
             uint32_t j0, jl, W, H, hl;
             uint64_t hb;
@@ -370,8 +430,12 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                 hl = (uint32_t)(hv >> kRangeShift);
                 srch = use_search(hl, rr.d, pl.bin_ratio);
             };
-            load_row();
-            for (uint32_t e = e0; e < e1; e++) {
+            auto restart = [&]() {
+                j = jfirst;
+                jn = sPre[j + 1];
+                load_row();
+            };
+            auto seek = [&](uint32_t e) {
                 if (e >= jn) {
                     do {
                         j++;
@@ -379,6 +443,24 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                     } while (e >= jn);
                     load_row();
                 }
+            };
+            uint32_t ent[kChunk];
+            restart();
+#pragma unroll
+            for (uint32_t k = 0; k < kChunk; k++) {
+                const uint32_t e = e0 + k;
+                ent[k] = 0;
+                if (e < e1) {
+                    seek(e);
+                    if (!srch) ent[k] = g.ocol[hb + (e - j0)];
+                }
+            }
+            restart();
+#pragma unroll
+            for (uint32_t k = 0; k < kChunk; k++) {
+                const uint32_t e = e0 + k;
+                if (e >= e1) break;
+                seek(e);
                 const uint32_t off = e - j0;
                 if (srch) {
                     // neighbour i of the root: is A[i] in A[j]'s larger-key range?
@@ -399,7 +481,7 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                     }
                     continue;
                 }
-                const uint32_t w = g.ocol[hb + off];
+                const uint32_t w = ent[k];
                 uint32_t s = hash_slot(w, H);
                 for (;;) {
                     const uint32_t key = sHKey[rr.hoff + s];
@@ -461,6 +543,42 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             return t;
         };
 
+        // Summed last level (pl.tail): F = the part of C_{L-1} fixed by
+        // x_1..x_{L-3}; for x_{L-2} = x, |C_{L-1}| = |F n (x, inf)| when
+        // L-2 is in smask[L-1], else |F \ {x}|.  Summed over x in S = C_{L-2}:
+        //   sum_w [ |S_w| (|F| - |F_{<=w}|) + pairs_w(S, F) ]  or  |S||F| - |S n F|
+        // with one group prefix sum of |F_w|.
+        auto tail_sum = [&](int l) -> uint64_t {
+            const int lv = l - 1;
+            const uint64_t S = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
+            uint64_t F = 0;
+            if (own) {
+                F = valid_mask(w, d);
+                uint32_t qm = sMisc[M_QM + l];
+                while (qm) {
+                    const int q = __builtin_ctz(qm);
+                    qm &= qm - 1;
+                    F &= E[xget(q) * W + w];
+                }
+                for (int s = 1; s < lv; s++) F &= ~bit_at(xget(s), w);
+                uint32_t sm = sMisc[M_SM + l] & ~(1u << lv);
+                while (sm) {
+                    const int s = __builtin_ctz(sm);
+                    sm &= sm - 1;
+                    F &= gt_mask(xget(s), w);
+                }
+            }
+            const uint32_t sc = (uint32_t)__popcll(S);
+            uint32_t fin = (uint32_t)__popcll(F);
+            for (int o = 1; o < P; o <<= 1) {
+                const uint32_t y = __shfl_up(fin, o, P);
+                if (lx >= o) fin += y;
+            }
+            const uint32_t ftot = __shfl(fin, P - 1, P);
+            if ((sMisc[M_SM + l] >> lv) & 1u) return (uint64_t)sc * (ftot - fin) + pairs_in_word(S, F);
+            return (uint64_t)sc * ftot - (uint64_t)__popcll(S & F);
+        };
+
         const uint64_t t2 = own ? level_word(2) : 0ull;
         if (L == 3) {
             cnt += (uint64_t)__popcll(t2);
@@ -474,6 +592,13 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             if (++spins == kClockEvery) {
                 spins = 0;
                 if (past(dl)) break;  // the next claim records the stop
+            }
+            if (pl.tail && l == L - 1) {  // every x_{L-2} of C_{L-2} at once
+                cnt += tail_sum(l);
+                if (l == 3) break;
+                --l;
+                pos = (uint32_t)xget(l - 1) + 1u;
+                continue;
             }
             uint64_t vv = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
             const uint32_t pw = pos >> 6;
@@ -583,8 +708,7 @@ __global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__r
 // Hash of every hub row: vertex -> row index (linear probing, 0 = empty).
 __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec *__restrict__ hubs,
                                                           const uint32_t *__restrict__ hub_of_row,
-                                                          uint64_t g0, uint64_t g1,
-                                                          uint32_t *keys, uint32_t *vals)
+                                                          uint64_t g0, uint64_t g1, uint64_t *kv)
 {
     for (uint64_t r = g0 + blockIdx.x * (uint64_t)kBlock + threadIdx.x; r < g1;
          r += (uint64_t)gridDim.x * kBlock) {
@@ -593,8 +717,10 @@ __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec
         const uint32_t v = g.ocol[h.rbeg + jl];
         uint32_t s = hash_slot(v, h.hslots);
         for (;;) {
-            if (atomicCAS(&keys[h.hoff + s], 0u, v + 1u) == 0u) {
-                vals[h.hoff + s] = jl;
+            // slot = key (low word) | row index (high word): a hit costs one load
+            uint32_t *slot = (uint32_t *)&kv[h.hoff + s];
+            if (atomicCAS(&slot[0], 0u, v + 1u) == 0u) {
+                slot[1] = jl;
                 break;
             }
             s = (s + 1) & (h.hslots - 1);
@@ -603,18 +729,22 @@ __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec
 }
 
 // Hub bitmaps: waves take chunks of 64 rows, scan the lengths of their
-// "higher" ranges and stream the concatenation lane-parallel.
+// "higher" ranges and stream the concatenation lane-parallel.  Rows below the
+// first kept row (rlo) only matter through kept columns, i.e. entries not
+// before the vertex of row rlo in the row order: a long range is cut to that
+// suffix by one binary search.  Each lane loads its kChunk entries back to
+// back, then their first hash slots back to back, then resolves the probes.
 __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubRec *__restrict__ hubs,
                                                             const uint32_t *__restrict__ hub_of_row,
                                                             uint64_t g0, uint64_t g1,
-                                                            const uint32_t *__restrict__ keys,
-                                                            const uint32_t *__restrict__ vals,
+                                                            const uint64_t *__restrict__ kv,
                                                             uint64_t *E, KOut out)
 {
     __shared__ uint64_t sBeg[kBlock];
     __shared__ uint32_t sPre[kBlock];
     __shared__ uint32_t sHub[kBlock];
     __shared__ uint32_t sJl[kBlock];
+    constexpr uint32_t kCutMin = 8;  // ranges this short are streamed whole
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t nchunks = (g1 - g0 + 63) / 64;
     const uint64_t wstride = (uint64_t)gridDim.x * (kBlock / 64);
@@ -632,8 +762,25 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
             const uint32_t jl = (uint32_t)(r - h.joff);
             const uint32_t v = g.ocol[h.rbeg + jl];
             const VInfo vi = g.vinfo[v];
-            sBeg[tid] = vi.hbeg;
+            uint64_t beg = vi.hbeg;
             len = vi.hlen;
+            if (jl < h.rlo) {
+                if (h.rlo >= h.d) {
+                    len = 0;  // no kept row: nothing to set from this row
+                } else if (len > kCutMin) {
+                    const uint32_t p = g.ocol[h.rbeg + h.rlo];
+                    const uint32_t dp = g.vinfo[p].deg;
+                    uint32_t lo = 0, hi2 = len;  // first entry not before p
+                    while (lo < hi2) {
+                        const uint32_t m = (lo + hi2) >> 1;
+                        const uint32_t x = g.ocol[beg + m];
+                        if (row_before(g.asc, g.vinfo[x].deg, x, dp, p)) lo = m + 1; else hi2 = m;
+                    }
+                    beg += lo;
+                    len -= lo;
+                }
+            }
+            sBeg[tid] = beg;
             sHub[tid] = hi;
             sJl[tid] = jl;
         }
@@ -648,13 +795,16 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
         // of the first, then the rows advance with the elements
         constexpr uint32_t kChunk = 8;
         for (uint32_t e0 = lane * kChunk; e0 < total; e0 += 64 * kChunk) {
-            int k = 0, hi = 63;  // smallest k with pre[k] > e0
-            while (k < hi) {
-                const int mid = (k + hi) >> 1;
-                if (sPre[wb + mid] > e0) hi = mid; else k = mid + 1;
+            int kfirst = 0;
+            {
+                int hi = 63;  // smallest k with pre[k] > e0
+                while (kfirst < hi) {
+                    const int mid = (kfirst + hi) >> 1;
+                    if (sPre[wb + mid] > e0) hi = mid; else kfirst = mid + 1;
+                }
             }
-            uint32_t kend = sPre[wb + k];  // elements of row k: [kbeg, kend)
-            uint32_t kbeg = k ? sPre[wb + k - 1] : 0u;
+            int k = 0;
+            uint32_t kbeg = 0, kend = 0;  // elements of row k: [kbeg, kend)
             HubRec h;
             uint32_t jl = 0;
             uint64_t W = 0, beg = 0;
@@ -664,9 +814,13 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                 W = (h.d + 63) >> 6;
                 beg = sBeg[wb + k];
             };
-            load_row();
-            const uint32_t e1 = e0 + kChunk < total ? e0 + kChunk : total;
-            for (uint32_t e = e0; e < e1; e++) {
+            auto restart = [&]() {
+                k = kfirst;
+                kend = sPre[wb + k];
+                kbeg = k ? sPre[wb + k - 1] : 0u;
+                load_row();
+            };
+            auto seek = [&](uint32_t e) {
                 if (e >= kend) {
                     do {
                         k++;
@@ -675,13 +829,45 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                     } while (e >= kend);
                     load_row();
                 }
-                const uint32_t wv2 = g.ocol[beg + (e - kbeg)];
+            };
+            const uint32_t e1 = e0 + kChunk < total ? e0 + kChunk : total;
+            uint32_t ent[kChunk];
+            restart();
+#pragma unroll
+            for (uint32_t q = 0; q < kChunk; q++) {
+                const uint32_t e = e0 + q;
+                ent[q] = 0;
+                if (e < e1) {
+                    seek(e);
+                    ent[q] = g.ocol[beg + (e - kbeg)];
+                }
+            }
+            // the first probe of every element, again back to back
+            uint64_t first[kChunk];
+            restart();
+#pragma unroll
+            for (uint32_t q = 0; q < kChunk; q++) {
+                const uint32_t e = e0 + q;
+                first[q] = 0;
+                if (e < e1) {
+                    seek(e);
+                    first[q] = kv[h.hoff + hash_slot(ent[q], h.hslots)];
+                }
+            }
+            restart();
+#pragma unroll
+            for (uint32_t q = 0; q < kChunk; q++) {
+                const uint32_t e = e0 + q;
+                if (e >= e1) break;
+                seek(e);
+                const uint32_t wv2 = ent[q];
                 uint32_t s = hash_slot(wv2, h.hslots);
+                uint64_t slot = first[q];
                 for (;;) {
-                    const uint32_t key = keys[h.hoff + s];
+                    const uint32_t key = (uint32_t)slot;
                     if (key == 0u) break;
                     if (key == wv2 + 1u) {
-                        const uint32_t i = vals[h.hoff + s];
+                        const uint32_t i = (uint32_t)(slot >> 32);
                         if (jl >= h.rlo)
                             atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(jl - h.rlo) * W + (i >> 6)],
                                      1ull << (i & 63));
@@ -690,6 +876,7 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                                      1ull << (jl & 63));
                     }
                     s = (s + 1) & (h.hslots - 1);
+                    slot = kv[h.hoff + s];
                 }
             }
         }
@@ -789,6 +976,62 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         return t;
     };
 
+    // Summed last level (pl.tail), l = L-1: adds |C_l| over every x_{l-1} in
+    // S = C_{l-1} n [pos0, hi2).  F = the part of C_l fixed by x_1..x_{l-2};
+    // with carry-chained wave prefix sums of |F_w| over the rounds,
+    //   sum_x |F n (x, inf)| = |S||F| - sum_w (|S_w| |F_{<=w}| - pairs_w(S, F))
+    //   sum_x |F \ {x}|      = |S||F| - |S n F|
+    // (words below both S's and F's first word add nothing to either).
+    const bool tail_gt = pl.tail && ((sSM[L - 1] >> (L - 2)) & 1u);
+    auto tail_count = [&](int l, uint32_t pos0, uint32_t hi2) {
+        const int lv = l - 1;
+        const uint32_t wl = sWlo[wv][l - 1];
+        const uint32_t ws = (pos0 >> 6) > wl ? (pos0 >> 6) : wl;  // first word of S
+        uint32_t fl = 0;  // first word where F can be non-zero
+        uint32_t fsm = sSM[l] & ~(1u << lv);
+        while (fsm) {
+            const int s = __builtin_ctz(fsm);
+            fsm &= fsm - 1;
+            const uint32_t v = xget(s) >> 6;
+            fl = v > fl ? v : fl;
+        }
+        const uint32_t w0 = tail_gt ? ws : (ws < fl ? ws : fl);
+        uint64_t acc = 0, sl = 0;
+        uint32_t carry = 0;
+        for (uint32_t rr = w0 >> 6; rr < R; rr++) {
+            const uint32_t w = rr * 64 + lane;
+            uint64_t S = 0, F = 0;
+            if (w < W && w >= w0) {
+                if (w >= ws) S = my[(uint64_t)(l - 3) * wmax + w] & ~lt_mask(pos0, w) & lt_mask(hi2, w);
+                if (w >= fl) {
+                    F = valid_mask(w, d);
+                    uint32_t qm = sQM[l];
+                    while (qm && F) {  // lanes whose word is already empty load nothing
+                        const int q = __builtin_ctz(qm);
+                        qm &= qm - 1;
+                        if (q == 1 && row1) F &= e1[w];
+                        else F &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
+                    }
+                    for (int s = 1; s < lv; s++) F &= ~bit_at(xget(s), w);
+                    uint32_t sm = sSM[l] & ~(1u << lv);
+                    while (sm) {
+                        const int s = __builtin_ctz(sm);
+                        sm &= sm - 1;
+                        F &= gt_mask(xget(s), w);
+                    }
+                }
+            }
+            const uint32_t sc = (uint32_t)__popcll(S);
+            const uint32_t fin = wave_incl_scan((uint32_t)__popcll(F)) + carry;
+            acc += tail_gt ? (uint64_t)sc * fin - pairs_in_word(S, F) : (uint64_t)__popcll(S & F);
+            sl += sc;
+            carry = __shfl(fin, 63, 64);
+        }
+        const uint64_t stot = wave_sum_u64(sl);
+        if (lane == 0) cnt += stot * (uint64_t)carry;
+        cnt -= acc;  // the wave's sum is exact modulo 2^64
+    };
+
     // DFS below level 2 with C_2 in my[0, W): x_2 runs over C_2 n [pos0, hi2).
     auto dfs3 = [&](uint32_t pos0, uint32_t hi2) {
         int l = 3;
@@ -797,6 +1040,13 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             if (++spins == kClockEvery) {
                 spins = 0;
                 if (past(dl)) break;  // the next claim records the stop
+            }
+            if (pl.tail && l == L - 1) {  // every x_{L-2} of the level at once
+                tail_count(l, pos, l == 3 ? hi2 : 0xffffffffu);
+                if (l == 3) break;
+                --l;
+                pos = xget(l - 1) + 1u;
+                continue;
             }
             uint32_t xi = 0xffffffffu;
             uint32_t pw = pos >> 6;
@@ -1022,11 +1272,11 @@ static void set_lds_attr(size_t lds)
 // Dispatch on the DFS group width.
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
-                              uint32_t b0, uint32_t b1, unsigned int *cursor, uint32_t ewords,
-                              const KOut &out, int *grid_out)
+                              uint32_t b0, uint32_t b1, unsigned int *cursor, const KOut &out,
+                              int *grid_out)
 {
     const uint32_t nlev = pl.L > 3 ? (uint32_t)(pl.L - 3) : 1u;
-    LdsLayout lay = make_lds_layout(ewords, nlev);
+    LdsLayout lay = make_lds_layout(lg, nlev);
     size_t lds = lay.bytes;
     int occ = 1;
     switch (lg) {
@@ -1089,7 +1339,7 @@ hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint6
 hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work,
                             uint32_t nwork, unsigned int *cursor, uint32_t nlev, int *grid_out)
 {
-    LdsLayout lay = make_lds_layout(LdsBudget::kMaxEWords, nlev);
+    LdsLayout lay = make_lds_layout(LdsBudget::kMaxClass, nlev);
     size_t lds = lay.bytes;
     (void)hipFuncSetAttribute((const void *)lds_multi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int occ = 0;
@@ -1104,18 +1354,17 @@ hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const
 }
 
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint32_t *keys, uint32_t *vals)
+                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv)
 {
-    hipLaunchKernelGGL(hub_hash_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, keys, vals);
+    hipLaunchKernelGGL(hub_hash_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, kv);
     return hipGetLastError();
 }
 
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint32_t *keys,
-                             const uint32_t *vals, uint64_t *E, const KOut &out)
+                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint64_t *kv, uint64_t *E,
+                             const KOut &out)
 {
-    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, keys,
-                       vals, E, out);
+    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, kv, E, out);
     return hipGetLastError();
 }
 

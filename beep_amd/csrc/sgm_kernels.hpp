@@ -26,6 +26,12 @@ struct KPlan {
     // bin_ratio x the root's degree looks the root's neighbours up in that
     // range by binary search instead of streaming it (0: always, ~0u: never)
     uint32_t bin_ratio;
+    // Count-only plans whose last level L-1 does not intersect with the row
+    // of x_{L-2}: C_{L-1} depends on x_{L-2} only through the exclusion of
+    // x_{L-2} and, when L-2 is in smask[L-1], the bound x_{L-1} > x_{L-2}.
+    // The kernels then sum |C_{L-1}| over all x_{L-2} in C_{L-2} in one pass
+    // (prefix popcounts) instead of walking x_{L-2}.
+    int32_t tail;
     uint32_t qmask[kMaxLevels];
     uint32_t smask[kMaxLevels];
 };

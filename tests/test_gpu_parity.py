@@ -328,3 +328,64 @@ def test_deep_pattern_on_hubs(cl_graph):
     assert pp.arrays()["n_hub_roots"] > 0
     assert r["count"] == o["count"]
     assert (r["hash_sum"], r["hash_xor"]) == (o["hash_sum"], o["hash_xor"])
+
+
+# Central-node patterns (index into patterns.central207()) whose last level
+# does not intersect with the row of the second-to-last position: count-only
+# runs sum that level in closed form.  4 and 5 vertices, with and without the
+# symmetry bound between the last two positions, and one of 6 vertices.
+TAIL_SUMMED = [2, 3, 4, 7, 8, 9, 10, 12, 13, 15, 20]
+
+
+def _central(i):
+    return P.central207()[i][1]
+
+
+def test_tail_summed_plans_are_the_ones_listed():
+    for i in TAIL_SUMMED:
+        p = B.Plan.from_edges(*P.directed(_central(i)), B.SGM)
+        assert p.width >= 4 and (p.width - 2) not in p.qedge[p.width - 1][1:], i
+
+
+@pytest.mark.parametrize("idx", TAIL_SUMMED)
+def test_summed_last_level_on_hubs(cl_graph, idx):
+    """Count-only runs (summed last level, LDS batches and hub items) equal the oracle."""
+    g, og = cl_graph
+    say("summed last level", idx)
+    r, o, pp = run_pair(g, og, _central(idx), B.SGM, emit=B.EMIT_NONE)
+    assert pp.arrays()["n_hub_roots"] > 0
+    assert r["count"] == o["count"], (r["count"], o["count"])
+
+
+@pytest.mark.parametrize("idx", [7, 8, 12, 20])
+@pytest.mark.parametrize("donate_min", [2, 16])
+def test_summed_last_level_donated(cl_graph, idx, donate_min):
+    """Deeper plans keep donating level-2 subtrees; the summed last level then
+    runs on the donated ranges too."""
+    g, og = cl_graph
+    say("summed last level donated", idx, donate_min)
+    s, d = P.directed(_central(idx))
+    pp = B.Prepared(g, B.Plan.from_edges(s, d, B.SGM))
+    pp.upload(0)
+    o = O.Prepared(og, O.plan(s, d, 0)).count(want_hash=False)
+    r = pp.count(emit=B.EMIT_NONE, donate=True, donate_min=donate_min)
+    assert r["donations"] > 0
+    assert r["count"] == o["count"]
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
+def test_summed_last_level_lane_widths(cl_graph, lanes):
+    g, og = cl_graph
+    say("summed last level lanes", lanes)
+    for idx in (4, 9):
+        r, o, _ = run_pair(g, og, _central(idx), B.SGM, emit=B.EMIT_NONE, lanes=lanes)
+        assert r["count"] == o["count"], idx
+
+
+@pytest.mark.parametrize("idx", [3, 4, 9, 13])
+def test_summed_last_level_degree_boundaries(idx):
+    g = star_plus_graph([63, 64, 65, 127, 128, 129, 255, 256, 511, 512, 513, 700, 1100])
+    og = oracle_graph(g)
+    say("summed last level boundaries", idx)
+    r, o, _ = run_pair(g, og, _central(idx), B.SGM, emit=B.EMIT_NONE)
+    assert r["count"] == o["count"]
