@@ -47,6 +47,9 @@ struct DevRoots {
     uint32_t nbatches = 0;
     HubRec *hubs = nullptr;
     uint32_t *hub_of_row = nullptr, *hub_of_item = nullptr;
+    // per hub row: the part of the row vertex's larger-key range the bitmap
+    // build streams (start | length << kRangeShift)
+    uint64_t *hub_range = nullptr;
     int skip_split = -1;  // plan property the hub items were built for
     uint32_t nhubs = 0;
     uint32_t wmax = 0;

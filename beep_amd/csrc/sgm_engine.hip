@@ -29,8 +29,8 @@ hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint6
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint64_t *kv, uint64_t *E,
-                             const KOut &out);
+                             const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
+                             const uint64_t *kv, uint64_t *E, const KOut &out);
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
@@ -68,11 +68,12 @@ Variant::~Variant() { delete dev; }
 void DevRoots::release()
 {
     for (void *p : {(void *)recs, (void *)batch_first, (void *)batch_count, (void *)hubs, (void *)hub_of_row,
-                    (void *)hub_of_item})
+                    (void *)hub_of_item, (void *)hub_range})
         if (p) (void)hipFree(p);
     recs = nullptr;
     batch_first = batch_count = hub_of_row = hub_of_item = nullptr;
     hubs = nullptr;
+    hub_range = nullptr;
 }
 
 void Lane::create(int dev, int ns)
@@ -154,6 +155,47 @@ static void upload_variant(Variant &V, int device)
     V.dev = dg;
 }
 
+// What hub_encode streams for every hub row j (vertex v = A[j]): v's
+// larger-key range.  A row below the first kept row (rlo) sets bits only in
+// kept columns, i.e. for entries not before A[rlo] in the row order (every
+// row is sorted by the same key), so its range is cut to that suffix; with
+// no kept row at all it is empty.
+static std::vector<uint64_t> hub_scan_ranges(const Variant &V, const std::vector<HubRec> &hubs)
+{
+    uint64_t rows = 0;
+    for (const auto &h : hubs) rows += h.d;
+    std::vector<uint64_t> out(rows);
+    auto before = [&](uint32_t a, uint32_t b) {  // a precedes b in a row
+        const uint32_t da = V.deg(a), db = V.deg(b);
+        if (da != db) return V.asc ? da < db : da > db;
+        return a < b;
+    };
+    parallel_for(hubs.size(), [&](size_t b, size_t e) {
+        for (size_t hi = b; hi < e; hi++) {
+            const HubRec &h = hubs[hi];
+            const uint32_t *A = V.ocol.data() + h.rbeg;
+            for (uint32_t j = 0; j < h.d; j++) {
+                const uint32_t v = A[j];
+                uint64_t beg = V.rowptr[v] + (V.asc ? V.nlow[v] : 0u);
+                uint64_t len = V.hlen[v];
+                if (j < h.rlo) {
+                    if (h.rlo >= h.d) {
+                        len = 0;
+                    } else {
+                        const uint32_t p = A[h.rlo];
+                        const uint32_t *rg = V.ocol.data() + beg;
+                        const uint64_t cut = std::partition_point(rg, rg + len, [&](uint32_t x) { return before(x, p); }) - rg;
+                        beg += cut;
+                        len -= cut;
+                    }
+                }
+                out[h.joff + j] = beg | (len << kRangeShift);
+            }
+        }
+    }, 1);
+    return out;
+}
+
 void upload(Prep &pp, int device, int p_skip_split, uint64_t arena_budget)
 {
     int ndev = 0;
@@ -218,12 +260,15 @@ void upload(Prep &pp, int device, int p_skip_split, uint64_t arena_budget)
             r.u = u;
             r.d = pp.wlen[h];
             r.split = pp.reduced ? 0u : V.nlow[u];
-            r.hslots = hash_slots_for(r.d);
             r.joff = joff;
             r.ioff = ioff;
             // x_1 < split items are skipped (sgm_kernelsLD.cuh:60): they are not listed.
             r.jbeg = (p_skip_split && r.split < r.d) ? r.split : (p_skip_split ? r.d : 0u);
             r.rlo = pp.rows_above_split ? std::min(r.split, r.d) : 0u;
+            // every pair the bitmap keeps has an endpoint in a kept row, and the
+            // scan ranges only hold entries not before A[rlo]: the hash needs the
+            // kept rows only
+            r.hslots = hash_slots_for(r.d - r.rlo);
             joff += r.d;
             ioff += r.d - r.jbeg;
             for (uint32_t j = 0; j < r.d; j++) hor.push_back(h);
@@ -265,6 +310,7 @@ void upload(Prep &pp, int device, int p_skip_split, uint64_t arena_budget)
             R.arena_words = std::max(R.arena_words, r.ewords);
             R.arena_slots = std::max(R.arena_slots, r.hslots);
         }
+        if (pp.uses_bitmap) R.hub_range = dev_upload(hub_scan_ranges(V, hubs));
         R.hubs = dev_upload(hubs);
         R.hub_of_row = dev_upload(hor);
         R.hub_of_item = dev_upload(hoi);
@@ -424,7 +470,7 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
             int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)cus * 16);
             HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv));
             int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
-            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv,
+            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, R.hub_range, r.g0, r.g1, s.kv,
                                         s.arena, c.out));
         }
         unsigned long long *cur = s.ctr + kCtrHub + (ri % 64);

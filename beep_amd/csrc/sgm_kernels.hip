@@ -66,10 +66,14 @@ __device__ __forceinline__ uint64_t lt_mask(uint32_t x, uint32_t w)
     return (1ull << (x & 63u)) - 1ull;
 }
 
-// Pairs (x, y) with x in S, y in F and y > x, inside one word; loops over the
-// sparser operand.
+// Pairs (x, y) with x in S, y in F and y > x, inside one word: C(|S|, 2) when
+// S = F, else a loop over the sparser operand.
 __device__ __forceinline__ uint32_t pairs_in_word(uint64_t S, uint64_t F)
 {
+    if (S == F) {
+        const uint32_t k = (uint32_t)__popcll(S);
+        return k * (k - 1) / 2;
+    }
     uint32_t n = 0;
     if (__popcll(S) <= __popcll(F)) {
         while (S) {
@@ -714,6 +718,7 @@ __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec
          r += (uint64_t)gridDim.x * kBlock) {
         const HubRec h = hubs[hub_of_row[r]];
         const uint32_t jl = (uint32_t)(r - h.joff);
+        if (jl < h.rlo) continue;  // rows below rlo are never found by a scan range
         const uint32_t v = g.ocol[h.rbeg + jl];
         uint32_t s = hash_slot(v, h.hslots);
         for (;;) {
@@ -728,14 +733,15 @@ __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec
     }
 }
 
-// Hub bitmaps: waves take chunks of 64 rows, scan the lengths of their
-// "higher" ranges and stream the concatenation lane-parallel.  Rows below the
-// first kept row (rlo) only matter through kept columns, i.e. entries not
-// before the vertex of row rlo in the row order: a long range is cut to that
-// suffix by one binary search.  Each lane loads its kChunk entries back to
-// back, then their first hash slots back to back, then resolves the probes.
+// Hub bitmaps: waves take chunks of 64 rows, scan the lengths of their scan
+// ranges (the larger-key range of the row vertex, cut on the host to the part
+// that can hold kept columns) and stream the concatenation lane-parallel.  A
+// chunk takes every nchunks-th row, so the long ranges of one hub's rows
+// spread over many waves.
+// Each lane loads its kChunk entries back to back before probing the hash.
 __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubRec *__restrict__ hubs,
                                                             const uint32_t *__restrict__ hub_of_row,
+                                                            const uint64_t *__restrict__ hub_range,
                                                             uint64_t g0, uint64_t g1,
                                                             const uint64_t *__restrict__ kv,
                                                             uint64_t *E, KOut out)
@@ -744,7 +750,6 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
     __shared__ uint32_t sPre[kBlock];
     __shared__ uint32_t sHub[kBlock];
     __shared__ uint32_t sJl[kBlock];
-    constexpr uint32_t kCutMin = 8;  // ranges this short are streamed whole
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint64_t nchunks = (g1 - g0 + 63) / 64;
     const uint64_t wstride = (uint64_t)gridDim.x * (kBlock / 64);
@@ -754,35 +759,15 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
             if (lane == 0) mark_stop(out);
             break;
         }
-        const uint64_t r = g0 + c * 64 + lane;
+        const uint64_t r = g0 + (uint64_t)lane * nchunks + c;  // rows strided over the chunks
         uint32_t len = 0;
         if (r < g1) {
             const uint32_t hi = hub_of_row[r];
-            const HubRec h = hubs[hi];
-            const uint32_t jl = (uint32_t)(r - h.joff);
-            const uint32_t v = g.ocol[h.rbeg + jl];
-            const VInfo vi = g.vinfo[v];
-            uint64_t beg = vi.hbeg;
-            len = vi.hlen;
-            if (jl < h.rlo) {
-                if (h.rlo >= h.d) {
-                    len = 0;  // no kept row: nothing to set from this row
-                } else if (len > kCutMin) {
-                    const uint32_t p = g.ocol[h.rbeg + h.rlo];
-                    const uint32_t dp = g.vinfo[p].deg;
-                    uint32_t lo = 0, hi2 = len;  // first entry not before p
-                    while (lo < hi2) {
-                        const uint32_t m = (lo + hi2) >> 1;
-                        const uint32_t x = g.ocol[beg + m];
-                        if (row_before(g.asc, g.vinfo[x].deg, x, dp, p)) lo = m + 1; else hi2 = m;
-                    }
-                    beg += lo;
-                    len -= lo;
-                }
-            }
-            sBeg[tid] = beg;
+            const uint64_t rg = hub_range[r];
+            sBeg[tid] = rg & ((1ull << kRangeShift) - 1ull);
+            len = (uint32_t)(rg >> kRangeShift);
             sHub[tid] = hi;
-            sJl[tid] = jl;
+            sJl[tid] = (uint32_t)(r - hubs[hi].joff);
         }
         const uint32_t inc = wave_incl_scan(len);
         sPre[tid] = inc;
@@ -842,18 +827,6 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                     ent[q] = g.ocol[beg + (e - kbeg)];
                 }
             }
-            // the first probe of every element, again back to back
-            uint64_t first[kChunk];
-            restart();
-#pragma unroll
-            for (uint32_t q = 0; q < kChunk; q++) {
-                const uint32_t e = e0 + q;
-                first[q] = 0;
-                if (e < e1) {
-                    seek(e);
-                    first[q] = kv[h.hoff + hash_slot(ent[q], h.hslots)];
-                }
-            }
             restart();
 #pragma unroll
             for (uint32_t q = 0; q < kChunk; q++) {
@@ -862,7 +835,7 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                 seek(e);
                 const uint32_t wv2 = ent[q];
                 uint32_t s = hash_slot(wv2, h.hslots);
-                uint64_t slot = first[q];
+                uint64_t slot = kv[h.hoff + s];
                 for (;;) {
                     const uint32_t key = (uint32_t)slot;
                     if (key == 0u) break;
@@ -881,6 +854,25 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
             }
         }
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Copies a W-word row (HBM) into a wave's buffer, lane i taking words i,
+// i + 64, ...; four loads per lane are in flight at a time.
+__device__ __forceinline__ void copy_row(uint64_t *dst, const uint64_t *__restrict__ src, uint32_t W, int lane)
+{
+    for (uint32_t w0 = 0; w0 < W; w0 += 4 * 64) {
+        uint64_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t w = w0 + k * 64 + lane;
+            v[k] = w < W ? src[w] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t w = w0 + k * 64 + lane;
+            if (w < W) dst[w] = v[k];
+        }
     }
 }
 
@@ -1110,11 +1102,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             sWlo[wv][2] = 0;
         }
         __builtin_amdgcn_wave_barrier();
-        if (row1)
-            for (uint32_t rr = 0; rr < R; rr++) {
-                const uint32_t w = rr * 64 + lane;
-                if (w < W) e1[w] = Eh[(uint64_t)(x1 - rlo) * W + w];
-            }
+        if (row1) copy_row(e1, Eh + (uint64_t)(x1 - rlo) * W, W, lane);
     };
 
     if (mode == 1) {
@@ -1135,10 +1123,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             const HubChunk ch = hq.q[c];
             if (ch.lo >= ch.hi) continue;
             bind(hubs[ch.hub], ch.x1);
-            for (uint32_t rr = 0; rr < R; rr++) {
-                const uint32_t w = rr * 64 + lane;
-                if (w < W) my[w] = hq.slots[ch.slot + w];
-            }
+            copy_row(my, hq.slots + ch.slot, W, lane);
             dfs3(ch.lo, ch.hi);
         }
         flush_acc(out, cnt, hs, hx);
@@ -1361,10 +1346,11 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
 }
 
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                             const uint32_t *hor, uint64_t g0, uint64_t g1, const uint64_t *kv, uint64_t *E,
-                             const KOut &out)
+                             const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
+                             const uint64_t *kv, uint64_t *E, const KOut &out)
 {
-    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, kv, E, out);
+    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, hrange, g0, g1, kv, E,
+                       out);
     return hipGetLastError();
 }
 
