@@ -18,11 +18,20 @@ namespace beep {
             throw Error(BEEP_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
-// Counter words of one launch sequence (u64): acc[4] | hub cursors[64] |
-// LDS cursors (32 x u32) | donation tail, head, slot tail, donated |
-// deadlines of the hub and the LDS kernels | stop flag.
-constexpr int kCtrAcc = 0, kCtrHub = 4, kCtrLds = 68, kCtrQTail = 84, kCtrQHead = 85, kCtrSlot = 86,
-              kCtrDonated = 87, kCtrDeadline = 88, kCtrStop = 90, kCtrWords = 91;
+// Counter words of one launch sequence (u64), in 128-byte lines of kLine
+// words.  Read back after the run (kCtrWords): the kStripes accumulator
+// stripes (kCtrAcc; see KOut::acc), then one line with the donation tail,
+// slot tail and donated count, the deadlines of the hub and the LDS kernels
+// and the stop flag.  Cleared but not read back: striped work cursors
+// (kCurWords each) of the hub rounds (64), the donation drain and the LDS
+// lane classes (32).
+constexpr int kCtrAcc = 0;
+constexpr int kCtrQTail = kStripes * kLine, kCtrSlot = kCtrQTail + 1, kCtrDonated = kCtrQTail + 2,
+              kCtrDeadline = kCtrQTail + 3, kCtrStop = kCtrQTail + 5;
+constexpr int kCtrWords = (kStripes + 1) * kLine;
+constexpr int kCurWords = kStripes * kLine;
+constexpr int kCtrHub = kCtrWords, kCtrHead = kCtrHub + 64 * kCurWords, kCtrLds = kCtrHead + kCurWords,
+              kCtrTotal = kCtrLds + 32 * kCurWords;
 
 // HBM copy of a variant: oriented rows and per-vertex records.
 struct DevGraph {

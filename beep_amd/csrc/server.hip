@@ -191,11 +191,13 @@ std::shared_ptr<Prep> Server::get_prep(const Plan &p, uint32_t sample_stride)
 }
 
 // Everything of a plan the enumeration depends on.
-static std::string plan_signature(const Plan &p)
+// Prediction key: the plan, and whether the query emits tuples (count-only
+// runs may sum the last level instead of walking it).
+static std::string plan_signature(const Plan &p, bool emits)
 {
     std::string s;
     auto put = [&](const void *b, size_t n) { s.append((const char *)b, n); };
-    const int head[] = {p.k, p.mode, p.lunmat, p.asc, p.max_qdeg, p.min_qdeg, p.L};
+    const int head[] = {p.k, p.mode, p.lunmat, p.asc, p.max_qdeg, p.min_qdeg, p.L, emits ? 1 : 0};
     put(head, sizeof(head));
     put(p.seq, sizeof(p.seq));
     put(p.qedge_ptr, sizeof(p.qedge_ptr));
@@ -221,7 +223,8 @@ static ScratchNeeds max_needs(const ScratchNeeds &a, const ScratchNeeds &b)
 beep_opts Server::pilot_opts(const beep_opts &o) const
 {
     beep_opts po = o;
-    po.emit = BEEP_EMIT_NONE;
+    // an emitting query walks every tuple: its pilot does too (hashes only)
+    po.emit = o.emit != BEEP_EMIT_NONE ? BEEP_EMIT_HASH : BEEP_EMIT_NONE;
     po.match_cap = 0;
     po.match_buf = nullptr;
     // hub phase: long enough for the finer-sampled hubs to reach kPilotSlack x
@@ -408,7 +411,7 @@ void Server::run()
                     j.pp = get_prep(j.plan);
                     j.need = scratch_needs(*j.pp, j.plan, j.opts, cus);
                     if (budget_ms > 0) {
-                        j.sig = plan_signature(j.plan);
+                        j.sig = plan_signature(j.plan, j.opts.emit != BEEP_EMIT_NONE);
                         auto it = predicted.find(j.sig);
                         if (it != predicted.end()) {
                             j.predicted_ms = it->second;

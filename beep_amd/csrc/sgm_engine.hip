@@ -22,7 +22,7 @@ namespace beep {
 
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
-                              uint32_t b0, uint32_t b1, unsigned int *cursor, const KOut &out,
+                              uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
                               int *grid_out);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
 hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st);
@@ -325,7 +325,7 @@ static uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
 
 uint64_t ScratchNeeds::bytes() const
 {
-    return align256(kCtrWords * 8) + align256(arena_words * 8) + align256(hash_slots * 8) +
+    return align256(kCtrTotal * 8) + align256(arena_words * 8) + align256(hash_slots * 8) +
            align256(stk_words * 8) + align256(queue_cap * sizeof(HubChunk)) + align256(slot_cap * 8) +
            align256(dbuf_words * 4) + align256(prof_words * 8);
 }
@@ -338,7 +338,7 @@ void carve(Scratch &s, char *base, const ScratchNeeds &n)
         off += align256(bytes);
         return p;
     };
-    s.ctr = (unsigned long long *)take(kCtrWords * 8);
+    s.ctr = (unsigned long long *)take(kCtrTotal * 8);
     s.arena = n.arena_words ? (uint64_t *)take(n.arena_words * 8) : nullptr;
     s.kv = n.hash_slots ? (uint64_t *)take(n.hash_slots * 8) : nullptr;
     s.stk = n.stk_words ? (uint64_t *)take(n.stk_words * 8) : nullptr;
@@ -433,7 +433,7 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
         c.hq.slots = s.slots;
         c.hq.slot_cap = s.cap.slot_cap;
         c.hq.tail = s.ctr + kCtrQTail;
-        c.hq.head = s.ctr + kCtrQHead;
+        c.hq.head = s.ctr + kCtrHead;
         c.hq.slot_tail = s.ctr + kCtrSlot;
         c.hq.donated = s.ctr + kCtrDonated;
         c.hq.split_min = o.donate_min ? o.donate_min : 64;
@@ -473,11 +473,14 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
             HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, R.hub_range, r.g0, r.g1, s.kv,
                                         s.arena, c.out));
         }
-        unsigned long long *cur = s.ctr + kCtrHub + (ri % 64);
-        if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, 8, st));
+        unsigned long long *cur = s.ctr + kCtrHub + (uint64_t)(ri % 64) * kCurWords;
+        if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, kCurWords * 8, st));
         const uint64_t items = r.i1 - r.i0;
         if (items) {
-            if (c.donate && ri > 0) HIP_CHECK(launch_fill_zero(s.ctr + kCtrQTail, 3 * 8, st));
+            if (c.donate && ri > 0) {  // queue tail, slot tail and the drain cursor start over
+                HIP_CHECK(launch_fill_zero(s.ctr + kCtrQTail, 2 * 8, st));
+                HIP_CHECK(launch_fill_zero(s.ctr + kCtrHead, kCurWords * 8, st));
+            }
             int g3 = (int)std::min<uint64_t>((items + 3) / 4, (uint64_t)c.hub_grid);
             HIP_CHECK(launch_hub_dfs(std::max(g3, 1), st, c.kg, c.kp, R.hubs, R.hub_of_item, r.i0, r.i1, cur,
                                      s.arena, s.stk, wmax, c.out, c.hq, 0));
@@ -501,7 +504,7 @@ static void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratc
         uint32_t e = b + 1;
         while (e < R.nbatches && std::max<int>(pp.batch_lanes_log2[e], c.lmin) == lg) e++;
         if (cls >= 32) throw Error(BEEP_ERANGE, "too many lane classes");
-        unsigned int *cur = (unsigned int *)(s.ctr + kCtrLds) + cls;
+        unsigned long long *cur = s.ctr + kCtrLds + (uint64_t)cls * kCurWords;
         HIP_CHECK(launch_lds_batches(lg, pp.dev->cus, streams[cls % nst], c.kg, c.kp, R.recs, R.batch_first,
                                      R.batch_count, b, e, cur, c.out, nullptr));
         b = e;
@@ -512,7 +515,7 @@ static void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratc
 void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane, unsigned long long *host_ctr)
 {
     hipStream_t st = lane.main;
-    HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, st));
+    HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrTotal, st));
     if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, st));
     HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
     HIP_CHECK(hipEventRecord(lane.ev[0], st));
@@ -531,11 +534,19 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
 
 void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res)
 {
-    res.count = acc[kCtrAcc + 0];
-    res.hash_sum = acc[kCtrAcc + 1];
-    res.hash_xor = acc[kCtrAcc + 2];
-    res.n_emitted = (o.emit != BEEP_EMIT_NONE) ? acc[0] : 0;
-    res.n_written = (o.emit == BEEP_EMIT_BUFFER && o.match_cap) ? std::min<uint64_t>(acc[3], o.match_cap) : 0;
+    uint64_t count = 0, hsum = 0, hxor = 0;
+    for (int t = 0; t < kStripes; t++) {
+        const unsigned long long *a = acc + kCtrAcc + t * kLine;
+        count += a[0];
+        hsum += a[1];
+        hxor ^= a[2];
+    }
+    res.count = count;
+    res.hash_sum = hsum;
+    res.hash_xor = hxor;
+    res.n_emitted = (o.emit != BEEP_EMIT_NONE) ? count : 0;
+    res.n_written =
+        (o.emit == BEEP_EMIT_BUFFER && o.match_cap) ? std::min<uint64_t>(acc[kCtrAcc + 3], o.match_cap) : 0;
     res.donations = acc[kCtrDonated];
     res.stopped = acc[kCtrStop] ? 1u : 0u;
 }
@@ -638,7 +649,7 @@ struct Batch {
     MultiWork *dwork = nullptr;
     uint32_t nwork = 0;
     uint32_t nlev = 1;
-    unsigned int *cursor = nullptr;
+    unsigned long long *cursor = nullptr;  // striped
     hipStream_t s0 = nullptr;
     std::vector<hipStream_t> branch;
     std::vector<hipEvent_t> done;
@@ -665,7 +676,7 @@ struct Batch {
 };
 
 hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work, uint32_t nwork,
-                            unsigned int *cursor, uint32_t nlev, int *grid_out);
+                            unsigned long long *cursor, uint32_t nlev, int *grid_out);
 
 Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opts *opts, int n)
 {
@@ -713,7 +724,7 @@ Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opt
         b->dqs = dev_upload(qs);
         b->dwork = dev_upload(w);
         b->nwork = (uint32_t)w.size();
-        b->cursor = dev_alloc<unsigned int>(1);
+        b->cursor = dev_alloc<unsigned long long>(kCurWords);
         HIP_CHECK(hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking));
         b->branch.resize(n + 1);
         b->done.resize(n + 1);
@@ -735,12 +746,12 @@ static void batch_capture(Batch &b)
     HIP_CHECK(hipStreamBeginCapture(b.s0, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < n; i++) {
         const Scratch &s = b.scr[i];
-        HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrWords, b.s0));
+        HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrTotal, b.s0));
         if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, b.s0));
         HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, b.cfgs[i].deadline_ticks, b.cfgs[i].deadline_ticks_lds,
                                       b.s0));
     }
-    HIP_CHECK(launch_fill_zero(b.cursor, 4, b.s0));
+    HIP_CHECK(launch_fill_zero(b.cursor, kCurWords * 8, b.s0));
     // The persistent LDS kernel runs first, on the whole GPU; the hub branches
     // follow it concurrently.  A persistent hub DFS dispatched while the LDS
     // kernel still holds every CU's LDS stays packed onto the CUs it got

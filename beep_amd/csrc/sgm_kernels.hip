@@ -155,17 +155,45 @@ __device__ __forceinline__ void prof_add(const KOut &out, int phase, uint32_t sl
                   (unsigned long long)((uint64_t)wall_clock64() - t0));
 }
 
-// Flushes per-lane accumulators: one atomic per wave and counter.
+// Flushes per-lane accumulators: one atomic per wave and counter, into the
+// wave's stripe.
 __device__ __forceinline__ void flush_acc(const KOut &out, uint64_t cnt, uint64_t hs, uint64_t hx)
 {
     cnt = wave_sum_u64(cnt);
     hs = wave_sum_u64(hs);
     hx = wave_xor_u64(hx);
     if ((threadIdx.x & 63) == 0) {
-        if (cnt) atomicAdd(&out.acc[0], (unsigned long long)cnt);
-        if (hs) atomicAdd(&out.acc[1], (unsigned long long)hs);
-        if (hx) atomicXor(&out.acc[2], (unsigned long long)hx);
+        const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+        unsigned long long *a = out.acc + (uint64_t)(wave % kStripes) * kLine;
+        if (cnt) atomicAdd(&a[0], (unsigned long long)cnt);
+        if (hs) atomicAdd(&a[1], (unsigned long long)hs);
+        if (hx) atomicXor(&a[2], (unsigned long long)hx);
     }
+}
+
+// Claims up to `step` consecutive indices of [0, n) from a striped cursor
+// (kStripes counters kLine words apart).  [0, n) is cut into kStripes parts;
+// a claimant starts at part `home` and moves on (k counts the parts it has
+// left) once a part is exhausted, which a plain load usually tells without
+// an atomic.  Returns false when every part is exhausted.
+__device__ __forceinline__ bool claim_striped(unsigned long long *cur, uint64_t n, uint32_t home, uint32_t &k,
+                                              uint64_t step, uint64_t &b, uint64_t &e)
+{
+    while (k < (uint32_t)kStripes) {
+        const uint32_t t = (home + k) % kStripes;
+        const uint64_t lo = n * t / kStripes, len = n * (t + 1) / kStripes - lo;
+        unsigned long long *p = cur + (uint64_t)t * kLine;
+        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len) {
+            const uint64_t o = atomicAdd(p, (unsigned long long)step);
+            if (o < len) {
+                b = lo + o;
+                e = o + step < len ? b + step : lo + len;
+                return true;
+            }
+        }
+        k++;
+    }
+    return false;
 }
 
 // Emits the tuples (u, A[x_1], ..., A[x_{l-1}], A[i]) for every bit i of word
@@ -641,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
                                                            const uint32_t *__restrict__ batch_first,
                                                            const uint32_t *__restrict__ batch_count,
                                                            uint32_t b0, uint32_t b1,
-                                                           unsigned int *cursor, LdsLayout lay,
+                                                           unsigned long long *cursor, LdsLayout lay,
                                                            KOut out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -654,13 +682,17 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
     uint64_t cnt = 0, hs = 0, hx = 0;
     const uint64_t dl = out.dl[1];
     const uint64_t t_start = wall_clock64();
+    uint32_t parts = 0;  // thread 0's claim state
     for (;;) {
         if (tid == 0) {
+            uint64_t bb = 0, be = 0;
             if (past(dl)) {
                 v.sMisc[M_BATCH] = b1;
                 mark_stop(out);
+            } else if (claim_striped(cursor, b1 - b0, blockIdx.x % kStripes, parts, 1, bb, be)) {
+                v.sMisc[M_BATCH] = b0 + (uint32_t)bb;
             } else {
-                v.sMisc[M_BATCH] = b0 + atomicAdd(cursor, 1u);
+                v.sMisc[M_BATCH] = b1;
             }
         }
         __syncthreads();
@@ -678,14 +710,19 @@ __global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
 // group width, then flushes its counts to that query's counters.
 __global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__restrict__ qs,
                                                            const MultiWork *__restrict__ work,
-                                                           uint32_t nwork, unsigned int *cursor,
+                                                           uint32_t nwork, unsigned long long *cursor,
                                                            LdsLayout lay)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const LdsView v = lds_view(smem, lay);
     const int tid = threadIdx.x;
+    uint32_t parts = 0;  // thread 0's claim state
     for (;;) {
-        if (tid == 0) v.sMisc[M_BATCH] = atomicAdd(cursor, 1u);
+        if (tid == 0) {
+            uint64_t bb = 0, be = 0;
+            v.sMisc[M_BATCH] = claim_striped(cursor, nwork, blockIdx.x % kStripes, parts, 1, bb, be)
+                                   ? (uint32_t)bb : nwork;
+        }
         __syncthreads();
         const uint32_t wi = v.sMisc[M_BATCH];
         if (wi >= nwork) break;
@@ -1108,15 +1145,13 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     if (mode == 1) {
         const uint64_t ntail = *hq.tail;
         const uint64_t nq = ntail < hq.cap ? ntail : hq.cap;
+        uint32_t parts = 0;  // lane 0's claim state
         for (;;) {
-            unsigned long long c = 0;
+            unsigned long long c = ~0ull;
             if (lane == 0) {
-                if (past(dl)) {
-                    c = ~0ull;
-                    mark_stop(out);
-                } else {
-                    c = atomicAdd(hq.head, 1ull);
-                }
+                uint64_t cb = 0, ce = 0;
+                if (past(dl)) mark_stop(out);
+                else if (claim_striped(hq.head, nq, wslot % kStripes, parts, 1, cb, ce)) c = cb;
             }
             c = __shfl(c, 0, 64);
             if (c >= nq) break;
@@ -1133,23 +1168,24 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
 
     constexpr uint32_t kClaim = 4;  // items per cursor claim
     uint64_t claim_end = 0, r = 0;
+    uint32_t parts = 0;  // lane 0's claim state
     for (;;) {
         if (r >= claim_end) {
-            unsigned long long it = 0;
+            unsigned long long cb = ~0ull, ce = 0;
             if (lane == 0) {
-                if (past(dl)) {
-                    it = ~0ull;
-                    mark_stop(out);
-                } else {
-                    it = atomicAdd(cursor, (unsigned long long)kClaim);
+                uint64_t b = 0, e = 0;
+                if (past(dl)) mark_stop(out);
+                else if (claim_striped(cursor, i1 - i0, wslot % kStripes, parts, kClaim, b, e)) {
+                    cb = b;
+                    ce = e;
                 }
             }
-            it = __shfl(it, 0, 64);
-            if (it == ~0ull) break;
-            r = i0 + it;
-            claim_end = r + kClaim < i1 ? r + kClaim : i1;
+            cb = __shfl(cb, 0, 64);
+            ce = __shfl(ce, 0, 64);
+            if (cb == ~0ull) break;
+            r = i0 + cb;
+            claim_end = i0 + ce;
         }
-        if (r >= i1) break;
         const uint64_t item = r++;
         const uint32_t hidx = hub_of_item[item];
         const HubRec h = hubs[hidx];
@@ -1230,7 +1266,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
 template <int P>
 static hipError_t launch_lds_p(int grid, size_t lds, hipStream_t st, const KGraph &g, const KPlan &pl,
                                const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
-                               uint32_t b0, uint32_t b1, unsigned int *cursor, const LdsLayout &lay,
+                               uint32_t b0, uint32_t b1, unsigned long long *cursor, const LdsLayout &lay,
                                const KOut &out)
 {
     hipLaunchKernelGGL(lds_batch_kernel<P>, dim3(grid), dim3(kBlock), lds, st, g, pl, recs, bf, bc, b0,
@@ -1257,7 +1293,7 @@ static void set_lds_attr(size_t lds)
 // Dispatch on the DFS group width.
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
-                              uint32_t b0, uint32_t b1, unsigned int *cursor, const KOut &out,
+                              uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
                               int *grid_out)
 {
     const uint32_t nlev = pl.L > 3 ? (uint32_t)(pl.L - 3) : 1u;
@@ -1322,7 +1358,7 @@ hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint6
 }
 
 hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work,
-                            uint32_t nwork, unsigned int *cursor, uint32_t nlev, int *grid_out)
+                            uint32_t nwork, unsigned long long *cursor, uint32_t nlev, int *grid_out)
 {
     LdsLayout lay = make_lds_layout(LdsBudget::kMaxClass, nlev);
     size_t lds = lay.bytes;

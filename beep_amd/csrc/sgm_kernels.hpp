@@ -11,6 +11,11 @@ namespace beep {
 constexpr int kBlock = 256;     // threads per workgroup (4 wave64s)
 constexpr int kMaxLevels = 10;  // query positions (DEPTH)
 
+// Device-wide atomics on one address serialise across the whole GPU, so
+// result accumulators and work cursors are spread over kStripes counters,
+// one per 128-byte line of kLine u64 words.
+constexpr int kLine = 16, kStripes = 16;
+
 // Query plan as the kernels consume it (by value).  Level l in [2, L) is
 // the candidate set C_l:
 //   C_l = {i in [0,d) \ {x_1..x_{l-1}} : E[x_q][i] = 1 for q in qmask[l],
@@ -89,7 +94,7 @@ struct HubChunk {
 struct HubQueue {
     HubChunk *q;                    // null: donation off
     unsigned long long *tail;       // records reserved
-    unsigned long long *head;       // records taken by the draining pass
+    unsigned long long *head;       // striped cursor of the draining pass
     uint64_t cap;
     uint64_t *slots;                // published C_2 bitmaps
     unsigned long long *slot_tail;  // words reserved
@@ -107,7 +112,9 @@ __host__ __device__ inline uint64_t hub_wave_words(int nlev, uint32_t wmax)
 }
 
 struct KOut {
-    unsigned long long *acc;  // [0] count [1] hash_sum [2] hash_xor [3] written
+    // kStripes lines: stripe s holds [0] count [1] hash_sum [2] hash_xor of
+    // the waves that flush to it; word 3 of stripe 0 counts tuples written
+    unsigned long long *acc;
     uint32_t *buf;            // device tuple buffer (cap * L words) or null
     uint64_t cap;
     // Deadlines in wall_clock64() ticks, written on the device by the launch

@@ -167,9 +167,11 @@ def test_server_deadline(c2_graph):
 def test_server_prediction_refuses_heavy_plans(c2_graph):
     say("server prediction chunglu1m")
     srv = B.Server(c2_graph, lanes=2, budget_ms=50.0)
-    heavy = plan_of("4star", B.SGM)    # sum over roots of C(d, 3) tuples, d up to 29k
+    # with tuple hashes every one of the sum over roots of C(d, 3) tuples
+    # (d up to 29k) is walked
+    heavy = plan_of("4star", B.SGM)
     light = plan_of("triangle", B.SGC)
-    srv.submit(heavy)
+    srv.submit(heavy, emit=B.EMIT_HASH)
     r1 = srv.poll(timeout_ms=120000)
     assert r1["status"] == B.ERANGE and r1["predicted_ms"] > 50.0
     srv.submit(light, emit=B.EMIT_HASH)
@@ -178,7 +180,7 @@ def test_server_prediction_refuses_heavy_plans(c2_graph):
     assert r2["status"] == 0 and 0.0 <= r2["predicted_ms"] <= 4 * 50.0
     assert (r2["count"], r2["hash_sum"], r2["hash_xor"]) == (ref["count"], ref["hash_sum"], ref["hash_xor"])
     pilots = srv.stats()["pilots"]
-    srv.submit(heavy)                  # refused from the cached prediction, no new pilot
+    srv.submit(heavy, emit=B.EMIT_HASH)  # refused from the cached prediction, no new pilot
     r3 = srv.poll(timeout_ms=120000)
     st = srv.stats()
     srv.close()
