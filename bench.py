@@ -11,8 +11,14 @@ Reported:
   roofline     B_alg / t_enum against 8 TB/s, B_alg = sum over roots of
                4(d(u)+2) + sum over neighbours 4(d(v)+2) (peeled degrees);
                t_enum is the HIP-event time of the query's launch sequence on
-               the library's stream; "traffic" is the measured HBM bytes per
-               step from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+               the library's stream.  B_alg charges every root with the whole
+               row of every neighbour; the kernels read only the larger-key
+               part of a neighbour's row, sum some last levels in closed form
+               and have re-read rows served by L2 / Infinity Cache, so
+               B_alg / t_enum can exceed the HBM peak.  "traffic" is the
+               measured HBM bytes per step (committed rocprofv3 FETCH_SIZE /
+               WRITE_SIZE summary) and "measured_gbs" / "measured_frac" the
+               HBM rate it gives over the same device time.
   cpu_baseline the oracle (CPU restatement of the reference enumeration,
                OpenMP over roots) on the same prepared graphs, all host cores
                up to 16, same two queries
@@ -232,7 +238,9 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "alg_bytes_per_step": alg_bytes},
+                     "alg_bytes_per_step": alg_bytes,
+                     "measured_gbs": traffic / (dev_ms_step / 1000.0) / 1e9 if traffic else None,
+                     "measured_frac": traffic / (dev_ms_step / 1000.0) / 1e9 / PEAK_HBM_GBS if traffic else None},
         "cpu_baseline": cpu,
         "parity": parity,
         "per_query_ms": {q: per_query_ms[i] / K for i, (q, _) in enumerate(queries)},
