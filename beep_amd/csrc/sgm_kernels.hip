@@ -171,23 +171,26 @@ __device__ __forceinline__ void flush_acc(const KOut &out, uint64_t cnt, uint64_
     }
 }
 
-// Claims up to `step` consecutive indices of [0, n) from a striped cursor
-// (kStripes counters kLine words apart).  [0, n) is cut into kStripes parts;
-// a claimant starts at part `home` and moves on (k counts the parts it has
-// left) once a part is exhausted, which a plain load usually tells without
-// an atomic.  Returns false when every part is exhausted.
+// Claims up to `step` indices of [0, n) from a striped cursor (kStripes
+// counters kLine words apart).  Part t of [0, n) holds the indices t,
+// t + kStripes, t + 2 kStripes, ..., so every part walks the work in its
+// order (heaviest first where the work is sorted so).  A claimant starts at
+// part `home` and moves on (k counts the parts it has left) once a part is
+// exhausted, which a plain load usually tells without an atomic.  On success
+// the claimed indices are first + q kStripes for q < count; returns false
+// when every part is exhausted.
 __device__ __forceinline__ bool claim_striped(unsigned long long *cur, uint64_t n, uint32_t home, uint32_t &k,
-                                              uint64_t step, uint64_t &b, uint64_t &e)
+                                              uint64_t step, uint64_t &first, uint64_t &count)
 {
     while (k < (uint32_t)kStripes) {
         const uint32_t t = (home + k) % kStripes;
-        const uint64_t lo = n * t / kStripes, len = n * (t + 1) / kStripes - lo;
+        const uint64_t len = t < n ? (n - t + kStripes - 1) / kStripes : 0;
         unsigned long long *p = cur + (uint64_t)t * kLine;
         if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len) {
             const uint64_t o = atomicAdd(p, (unsigned long long)step);
             if (o < len) {
-                b = lo + o;
-                e = o + step < len ? b + step : lo + len;
+                first = t + o * kStripes;
+                count = o + step < len ? step : len - o;
                 return true;
             }
         }
@@ -1167,26 +1170,28 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     }
 
     constexpr uint32_t kClaim = 4;  // items per cursor claim
-    uint64_t claim_end = 0, r = 0;
+    uint64_t nxt = 0, left = 0;
     uint32_t parts = 0;  // lane 0's claim state
     for (;;) {
-        if (r >= claim_end) {
-            unsigned long long cb = ~0ull, ce = 0;
+        if (left == 0) {
+            unsigned long long cb = ~0ull, cc = 0;
             if (lane == 0) {
-                uint64_t b = 0, e = 0;
+                uint64_t b = 0, n = 0;
                 if (past(dl)) mark_stop(out);
-                else if (claim_striped(cursor, i1 - i0, wslot % kStripes, parts, kClaim, b, e)) {
+                else if (claim_striped(cursor, i1 - i0, wslot % kStripes, parts, kClaim, b, n)) {
                     cb = b;
-                    ce = e;
+                    cc = n;
                 }
             }
             cb = __shfl(cb, 0, 64);
-            ce = __shfl(ce, 0, 64);
+            cc = __shfl(cc, 0, 64);
             if (cb == ~0ull) break;
-            r = i0 + cb;
-            claim_end = i0 + ce;
+            nxt = i0 + cb;
+            left = cc;
         }
-        const uint64_t item = r++;
+        const uint64_t item = nxt;
+        nxt += kStripes;
+        left--;
         const uint32_t hidx = hub_of_item[item];
         const HubRec h = hubs[hidx];
         const uint32_t jl = h.jbeg + (uint32_t)(item - h.ioff);
