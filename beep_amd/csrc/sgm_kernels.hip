@@ -175,26 +175,31 @@ __device__ __forceinline__ void flush_acc(const KOut &out, uint64_t cnt, uint64_
 // counters kLine words apart).  Part t of [0, n) holds the indices t,
 // t + kStripes, t + 2 kStripes, ..., so every part walks the work in its
 // order (heaviest first where the work is sorted so).  A claimant starts at
-// part `home` and moves on (k counts the parts it has left) once a part is
-// exhausted, which a plain load usually tells without an atomic.  On success
-// the claimed indices are first + q kStripes for q < count; returns false
-// when every part is exhausted.
+// part `home` and moves on once a part is exhausted; on entering a part, a
+// plain load usually tells an exhausted one without an atomic.  k (start at
+// 0) holds the parts left behind and whether the current part has been
+// claimed from.  On success the claimed indices are first + q kStripes for
+// q < count; returns false when every part is exhausted.
+constexpr uint32_t kPartEntered = 1u << 16;
 __device__ __forceinline__ bool claim_striped(unsigned long long *cur, uint64_t n, uint32_t home, uint32_t &k,
                                               uint64_t step, uint64_t &first, uint64_t &count)
 {
-    while (k < (uint32_t)kStripes) {
-        const uint32_t t = (home + k) % kStripes;
+    while ((k & 0xffffu) < (uint32_t)kStripes) {
+        const uint32_t t = (home + (k & 0xffffu)) % kStripes;
         const uint64_t len = t < n ? (n - t + kStripes - 1) / kStripes : 0;
         unsigned long long *p = cur + (uint64_t)t * kLine;
-        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len) {
-            const uint64_t o = atomicAdd(p, (unsigned long long)step);
-            if (o < len) {
-                first = t + o * kStripes;
-                count = o + step < len ? step : len - o;
-                return true;
-            }
+        if (!(k & kPartEntered) && __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) {
+            k = (k & 0xffffu) + 1;
+            continue;
         }
-        k++;
+        k |= kPartEntered;
+        const uint64_t o = atomicAdd(p, (unsigned long long)step);
+        if (o < len) {
+            first = t + o * kStripes;
+            count = o + step < len ? step : len - o;
+            return true;
+        }
+        k = (k & 0xffffu) + 1;
     }
     return false;
 }
