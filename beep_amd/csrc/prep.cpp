@@ -124,38 +124,29 @@ std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc_flag
     for (uint32_t sl = 0; sl < nslices; sl++) {
     const size_t s0 = (size_t)n * sl / nslices, s1 = (size_t)n * (sl + 1) / nslices;
     parallel_for(s1 - s0, [&](size_t b, size_t e) {
+        // rows are sorted on 64-bit keys (degree, or its complement for desc
+        // rows, above the id): the order of the comparison on (deg, id)
+        // without a degree lookup per comparison
+        std::vector<uint64_t> key;
         for (size_t u = s0 + b; u < s0 + e; u++) {
             const uint64_t rb = rp[u], re = rp[u + 1];
-            uint32_t du = deg[u], lo = 0;
+            const uint32_t du = deg[u];
+            uint32_t lo = 0, hc = 0;
+            key.resize(re - rb);
             for (uint64_t x = rb; x < re; x++) {
-                uint32_t w = V.col[x];
-                if (deg[w] < du || (deg[w] == du && w < (uint32_t)u)) lo++;
-                V.ocol[x] = w;
+                const uint32_t w = V.col[x], dw = deg[w];
+                if (dw < du || (dw == du && w < (uint32_t)u)) lo++;
+                if (dw > du || (dw == du && w < (uint32_t)u)) hc++;
+                key[x - rb] = (uint64_t)(asc ? dw : ~dw) << 32 | w;
             }
+            std::sort(key.begin(), key.end());
+            for (uint64_t x = rb; x < re; x++) V.ocol[x] = (uint32_t)key[x - rb];
             V.nlow[u] = lo;
             // Bitmap-build scan range: every adjacent pair {v, w} is found from
             // exactly one endpoint.  asc rows: the suffix of larger keys.  desc
             // rows (degree descending, ids ascending): the prefix of larger
             // degrees followed by equal degrees with smaller ids.
-            if (asc) {
-                V.hlen[u] = du - lo;
-            } else {
-                uint32_t c = 0;
-                for (uint64_t x = rb; x < re; x++) {
-                    uint32_t w = V.col[x];
-                    if (deg[w] > du || (deg[w] == du && w < (uint32_t)u)) c++;
-                }
-                V.hlen[u] = c;
-            }
-            uint32_t *row = V.ocol.data() + rb;
-            if (asc)
-                std::sort(row, row + (re - rb), [&](uint32_t a, uint32_t c) {
-                    return deg[a] != deg[c] ? deg[a] < deg[c] : a < c;
-                });
-            else
-                std::sort(row, row + (re - rb), [&](uint32_t a, uint32_t c) {
-                    return deg[a] != deg[c] ? deg[a] > deg[c] : a < c;
-                });
+            V.hlen[u] = asc ? du - lo : hc;
         }
     }, 4096);
     if (nslices > 1) progress("orientation: rows %zu..%zu done", s0, s1);
