@@ -45,6 +45,10 @@ int peel(const Graph &g, int min_qdeg, std::vector<uint64_t> &rp, std::vector<ui
         std::vector<uint32_t> S;
         for (uint32_t v = 0; v < n; v++)
             if (deg[v] >= 1 && deg[v] < lim) S.push_back(v);
+        if (S.empty()) {  // the round removes nothing: the rows stay as they are
+            progress("peel: round %d removed the edges of 0 vertices", rounds);
+            break;
+        }
         const uint64_t m = rp[n];
         keep.assign(m, 1);
         const bool sorted = g.rows_sorted;
