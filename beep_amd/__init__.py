@@ -160,17 +160,31 @@ class Prepared:
 
     def arrays(self):
         d = self.describe()
-        A = np.ctypeslib.as_array
         n, m = d.n, d.m
-        return dict(rowptr=A(d.rowptr, shape=(n + 1,)).copy(),
-                    col=A(d.col, shape=(max(m, 1),))[:m].copy(),
-                    ocol=A(d.ocol, shape=(max(m, 1),))[:m].copy(),
-                    nlow=A(d.nlow, shape=(max(n, 1),))[:n].copy(),
-                    roots=A(d.roots, shape=(max(d.n_roots, 1),))[:d.n_roots].copy(),
+
+        def A(ptr, k, dtype):  # host view of k entries (an empty vector may be NULL)
+            if k == 0 or not ptr:
+                return np.zeros(0, dtype=dtype)
+            return np.ctypeslib.as_array(ptr, shape=(k,)).copy()
+
+        return dict(rowptr=A(d.rowptr, n + 1, np.uint64),
+                    col=A(d.col, m, np.uint32),
+                    ocol=A(d.ocol, m, np.uint32),
+                    nlow=A(d.nlow, n, np.uint32),
+                    roots=A(d.roots, d.n_roots, np.uint32),
                     q0_size=d.q0_size, boundary=d.boundary, cutoff=d.cutoff,
                     max_deg=d.max_degree, peel_rounds=d.peel_rounds, asc=d.asc,
                     n_hub_roots=d.n_hub_roots, n_lds_batches=d.n_lds_batches,
-                    alg_bytes=d.alg_bytes, max_root_degree=d.max_root_degree)
+                    alg_bytes=d.alg_bytes, max_root_degree=d.max_root_degree,
+                    p1=bool(d.p1), p2=bool(d.p2), q0_head=d.q0_head, q0_head_larger=d.q0_head_larger)
+
+    def preconditions(self):
+        """P1 (no vertex above the root window) and P2 (the head skip drops
+        nothing), with the figures they are decided on."""
+        d = self.describe()
+        return dict(p1=bool(d.p1), p2=bool(d.p2), max_degree=d.max_degree, cutoff=d.cutoff,
+                    window_hi=100 * d.cutoff, boundary=d.boundary, asc=bool(d.asc),
+                    q0_head=d.q0_head, q0_head_larger=d.q0_head_larger)
 
     def upload(self, device=0):
         check(lib().beep_prep_upload(self._h, device))

@@ -30,7 +30,8 @@ class PrepDesc(C.Structure):
                 ("max_degree", C.c_uint32), ("max_root_degree", C.c_uint32),
                 ("peel_rounds", C.c_int), ("asc", C.c_int), ("alg_bytes", C.c_uint64),
                 ("n_hub_roots", C.c_uint64), ("n_lds_batches", C.c_uint64),
-                ("rowptr", u64p), ("col", u32p), ("ocol", u32p), ("nlow", u32p), ("roots", u32p)]
+                ("rowptr", u64p), ("col", u32p), ("ocol", u32p), ("nlow", u32p), ("roots", u32p),
+                ("p1", C.c_int), ("p2", C.c_int), ("q0_head", C.c_uint32), ("q0_head_larger", C.c_uint32)]
 
 
 class Opts(C.Structure):

@@ -275,6 +275,10 @@ int beep_prep_describe(const beep_prep *pp, beep_prep_desc *o)
         o->ocol = V.ocol.data();
         o->nlow = V.nlow.data();
         o->roots = P.roots.data();
+        o->p1 = (uint64_t)V.max_degree < (uint64_t)V.cutoff * 100u ? 1 : 0;
+        o->p2 = (!V.asc || P.boundary == 0 || P.q0_head_larger == 0) ? 1 : 0;
+        o->q0_head = P.q0_head;
+        o->q0_head_larger = P.q0_head_larger;
     });
 }
 

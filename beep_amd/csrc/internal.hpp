@@ -151,6 +151,8 @@ struct Prep {
     uint32_t max_root_degree = 0;
     std::vector<uint32_t> roots;   // R
     uint64_t q0_size = 0, boundary = 0;
+    uint32_t q0_head = ~0u;        // Q0[0]
+    uint32_t q0_head_larger = 0;   // its neighbours with a larger key
     uint64_t alg_bytes = 0;
 
     // Work list: the roots that carry work, with the part of the oriented row

@@ -102,6 +102,14 @@ int peel(const Graph &g, int min_qdeg, std::vector<uint64_t> &rp, std::vector<ui
 
 std::shared_ptr<Variant> make_variant(const Graph &g, int min_qdeg, int asc_flag)
 {
+    // The reference's bitmap build binary-searches A[i] in the id-ordered row
+    // of A[j] (host_utils.cuh:284-316), and its orientation keeps the input's
+    // order among neighbours of equal degree (a stable sort on degree only,
+    // subgraph_matching.cuh:1305-1331): its counts are defined only for rows
+    // sorted by id, which the reference's -s option produces (main.cu:88-91).
+    if (!g.rows_sorted)
+        throw Error(BEEP_EINVAL, "graph rows are not sorted by destination id; load the graph with "
+                                 "edge sorting on (-s 1, beep_graph_load(path, 1, ...))");
     auto vp = std::make_shared<Variant>();
     Variant &V = *vp;
     V.n = g.n;
@@ -183,7 +191,7 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
     // Roots: Q0 = {max_qdeg <= d < 100 cutoff} ordered by degree descending
     // with ids ascending; boundary = #{d > cutoff}; the head of Q0 is skipped
     // when ascending and boundary >= 1 (work_list_head1 starts at 1).
-    const uint32_t hi = V.cutoff * 100u;
+    const uint64_t hi = (uint64_t)V.cutoff * 100u;
     const uint32_t lo = (uint32_t)p.max_qdeg;
     std::vector<uint64_t> bucket((size_t)maxd + 2, 0);
     uint64_t nq = 0;
@@ -200,6 +208,11 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
     pp->roots.assign(q0.begin() + (ptrdiff_t)skip, q0.end());
     pp->q0_size = nq;
     pp->boundary = bnd;
+    if (nq) {
+        const uint32_t h = q0[0];
+        pp->q0_head = h;
+        pp->q0_head_larger = deg(h) - V.nlow[h];
+    }
     pp->max_root_degree = pp->roots.empty() ? 0 : deg(pp->roots[0]);
     // B_alg = sum over roots of 4(d(u)+2) + sum over neighbours 4(d(v)+2).
     std::atomic<uint64_t> bytes{0};

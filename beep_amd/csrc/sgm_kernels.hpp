@@ -91,17 +91,18 @@ struct HubChunk {
     uint32_t lo, hi;
 };
 
+// Every field defaults to null / 0, i.e. donation off, unless set on purpose.
 struct HubQueue {
-    HubChunk *q;                    // null: donation off
-    unsigned long long *tail;       // records reserved
-    unsigned long long *head;       // striped cursor of the draining pass
-    uint64_t cap;
-    uint64_t *slots;                // published C_2 bitmaps
-    unsigned long long *slot_tail;  // words reserved
-    uint64_t slot_cap;
-    unsigned long long *donated;    // records published over the whole call
-    uint32_t split_min;             // donate when |C_2| > split_min
-    uint32_t chunk;                 // candidates per record
+    HubChunk *q = nullptr;                    // null: donation off
+    unsigned long long *tail = nullptr;       // records reserved
+    unsigned long long *head = nullptr;       // striped cursor of the draining pass
+    uint64_t cap = 0;
+    uint64_t *slots = nullptr;                // published C_2 bitmaps
+    unsigned long long *slot_tail = nullptr;  // words reserved
+    uint64_t slot_cap = 0;
+    unsigned long long *donated = nullptr;    // records published over the whole call
+    uint32_t split_min = 0;                   // donate when |C_2| > split_min
+    uint32_t chunk = 0;                       // candidates per record
 };
 
 // Words (u64) of one hub-DFS wave's buffer: nlev candidate-stack rows and the
@@ -114,20 +115,20 @@ __host__ __device__ inline uint64_t hub_wave_words(int nlev, uint32_t wmax)
 struct KOut {
     // kStripes lines: stripe s holds [0] count [1] hash_sum [2] hash_xor of
     // the waves that flush to it; word 3 of stripe 0 counts tuples written
-    unsigned long long *acc;
-    uint32_t *buf;            // device tuple buffer (cap * L words) or null
-    uint64_t cap;
+    unsigned long long *acc = nullptr;
+    uint32_t *buf = nullptr;  // device tuple buffer (cap * L words) or null
+    uint64_t cap = 0;
     // Deadlines in wall_clock64() ticks, written on the device by the launch
     // sequence's first kernel (all ones: none): dl[0] for the hub kernels,
     // dl[1] for the LDS kernels.  Kernels stop claiming work and leave their
     // DFS once theirs has passed, and set *stop.
-    const unsigned long long *dl;
-    unsigned long long *stop;
+    const unsigned long long *dl = nullptr;
+    unsigned long long *stop = nullptr;
     // Optional busy-time accumulators (wall_clock64 ticks), kProfPhases x
     // kProfSlots: phase 0 LDS workgroups, 1 hub waves on their own items, 2 hub
     // waves on donated subtrees, 3 the LDS workgroups' staging and bitmap
     // build; slot = workgroup (LDS) or wave (hub) index.
-    unsigned long long *prof;
+    unsigned long long *prof = nullptr;
 };
 
 constexpr int kProfSlots = 4096, kProfPhases = 4;

@@ -150,6 +150,16 @@ typedef struct {
     const uint32_t *ocol;    /* peeled, degree oriented */
     const uint32_t *nlow;    /* #neighbours with smaller (degree, id) key */
     const uint32_t *roots;   /* R in processing order */
+    /* Preconditions under which the counts are the true counts of the
+     * pattern (count_subgraphs' root window and head skip,
+     * subgraph_matching.cuh:1052, 1081-1124, 1201; device_utils.cuh:66-94):
+     *   p1: max peeled degree < 100 * cutoff, i.e. no vertex is left out of
+     *       the root window by its upper bound;
+     *   p2: the head skip drops nothing -- the plan is not ascending, or
+     *       boundary = 0, or Q0[0] has no neighbour with a larger key. */
+    int p1, p2;
+    uint32_t q0_head;        /* Q0[0] (UINT32_MAX when Q0 is empty) */
+    uint32_t q0_head_larger; /* neighbours of Q0[0] with a larger (degree, id) key */
 } beep_prep_desc;
 
 int beep_prep_describe(const beep_prep *pp, beep_prep_desc *out);

@@ -397,6 +397,12 @@ int or_prepare(const or_csr *g, const or_plan *p, uint32_t cutoff, or_prep *out)
     memset(out, 0, sizeof(*out));
     uint32_t n = g->n;
     uint64_t m = g->m;
+    /* The bitmap build binary-searches id-ordered rows (host_utils.cuh:284-316)
+     * and the orientation keeps the input order among equal degrees
+     * (subgraph_matching.cuh:1305-1331): rows must be sorted by id (-s). */
+    for (uint32_t v = 0; v < n; v++)
+        for (uint64_t e = g->rowptr[v] + 1; e < g->rowptr[v + 1]; e++)
+            if (g->col[e - 1] > g->col[e]) return -3;
     uint64_t *rp = malloc((n + 1) * sizeof(uint64_t));
     uint32_t *col = malloc((m ? m : 1) * sizeof(uint32_t));
     uint32_t *deg = malloc((n ? n : 1) * sizeof(uint32_t));

@@ -389,3 +389,27 @@ def test_summed_last_level_degree_boundaries(idx):
     say("summed last level boundaries", idx)
     r, o, _ = run_pair(g, og, _central(idx), B.SGM, emit=B.EMIT_NONE)
     assert r["count"] == o["count"]
+
+
+def test_golden_quirk_cases():
+    """Root selection where it decides the count (tests/golden/quirks.json:
+    a vertex at the root window's upper bound, the ascending head skip with
+    b >= 1, regular graphs with b = 0): the HIP path equals the brute force."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "quirks.json")))
+    graphs = {}
+    for name, v in gold["graphs"].items():
+        e = sorted({(a, b) for a, b in v["edges"]} | {(b, a) for a, b in v["edges"]})
+        graphs[name] = B.Graph.from_edges([a for a, _ in e], [b for _, b in e])
+    bad = []
+    say("golden quirk cases", len(gold["cases"]))
+    for c in gold["cases"]:
+        s, d = P.directed(gold["patterns"][c["pattern"]])
+        pp = B.Prepared(graphs[c["graph"]], B.Plan.from_edges(s, d, c["mode"]))
+        pp.upload(0)
+        for emit in (B.EMIT_NONE, B.EMIT_HASH):
+            got = pp.count(emit=emit)["count"]
+            if got != c["count"]:
+                bad.append((c["graph"], c["pattern"], c["mode"], emit, got, c["count"]))
+    assert not bad, bad[:10]

@@ -155,3 +155,29 @@ def test_load_errors():
     assert e.value.code == -22
     with pytest.raises(B.BeepError):
         B.Graph.from_edges([1, 0], [0, 1])  # not grouped by ascending source
+
+
+def test_unsorted_rows_are_refused():
+    """Rows out of id order (a .bel read without -s) are refused by both the
+    product and the oracle: the reference's bitmap build binary-searches the
+    id-ordered row (host_utils.cuh:284-316), so its counts are defined only
+    for sorted rows.  The same edges sorted are accepted."""
+    import numpy as np
+    import oracle_py as O
+    tri = [(0, 1), (0, 2), (1, 2), (1, 3), (2, 3)]
+    e = sorted({(a, b) for a, b in tri} | {(b, a) for a, b in tri})
+    src = [a for a, _ in e]
+    dst = [b for _, b in e]
+    # row 1 = [0, 2, 3] written as [3, 0, 2]
+    i = src.index(1)
+    dst[i:i + 3] = [3, 0, 2]
+    g = B.Graph.from_edges(src, dst)
+    plan = B.Plan.from_undirected([(0, 1), (1, 2), (0, 2)], "sgc")
+    with pytest.raises(B.BeepError) as ei:
+        B.Prepared(g, plan)
+    assert "not sorted" in str(ei.value)
+    s, d = P.directed([(0, 1), (1, 2), (0, 2)])
+    with pytest.raises(ValueError):
+        O.Prepared(O.Graph(np.array(src, dtype=np.uint32), np.array(dst, dtype=np.uint32)), O.plan(s, d, 1))
+    gs = B.Graph.from_edges([a for a, _ in e], [b for _, b in e])
+    assert B.Prepared(gs, plan).arrays()["q0_size"] == 4
