@@ -209,6 +209,27 @@ class Prepared:
             out["matches"] = buf[:r.n_written * self.plan.width].reshape(-1, self.plan.width)
         return out
 
+    def count_noalloc(self, device=0, emit=EMIT_NONE, match_cap=0, lanes=0, donate=True, donate_min=0,
+                      deadline_ms=0.0, build_mode=0, stream=None):
+        """beep_count_noalloc: the same enumeration with its scratch taken
+        from the library's fixed device array (no device allocation); the
+        prepared query must have been uploaded.  ``stream`` is a hipStream_t
+        as an int (None: the null stream)."""
+        o = _opts(device, emit, lanes, donate, donate_min, deadline_ms, False, build_mode)
+        buf = None
+        if emit == EMIT_BUFFER and match_cap:
+            buf = np.zeros(match_cap * self.plan.width, dtype=np.uint32)
+            o.match_cap = match_cap
+            o.match_buf = buf.ctypes.data_as(_ffi.u32p)
+        r = _ffi.Result()
+        rc = lib().beep_count_noalloc(self._h, self.plan._h, C.byref(o), C.c_void_p(stream), C.byref(r))
+        if rc != ETIMEDOUT:
+            check(rc)
+        out = _result_dict(r)
+        if buf is not None:
+            out["matches"] = buf[:r.n_written * self.plan.width].reshape(-1, self.plan.width)
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -89,6 +89,7 @@ SIGNATURES = {
     "beep_prep_free": (None, [C.c_void_p]),
     "beep_opts_default": (None, [C.POINTER(Opts)]),
     "beep_count": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.POINTER(Result)]),
+    "beep_count_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Opts), C.c_void_p, C.POINTER(Result)]),
     "beep_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(Opts), C.c_int,
                                     C.POINTER(C.c_void_p)]),
     "beep_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Result), C.POINTER(C.c_double)]),

@@ -43,6 +43,7 @@ struct beep_server { beep::Server *s; };
 namespace beep {
 
 void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &res);
+void count_noalloc(Prep &pp, const Plan &p, const beep_opts &o, void *stream, beep_result &res);
 void upload_prep(Prep &pp, int device, int skip_split);
 
 static thread_local std::string g_last_error;
@@ -314,6 +315,20 @@ int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_resul
         check_opts(opt);
         memset(out, 0, sizeof(*out));
         beep::count_on_device(*pp->pp, *p->p, opt, *out);
+        if (out->stopped) throw Error(BEEP_ETIMEDOUT, "deadline passed: the counts are partial");
+    });
+}
+
+int beep_count_noalloc(beep_prep *pp, const beep_plan *p, const beep_opts *o, void *stream, beep_result *out)
+{
+    return guarded([&] {
+        if (!pp || !p || !out) throw Error(BEEP_EINVAL, "null argument");
+        beep_opts def;
+        beep_opts_default(&def);
+        const beep_opts &opt = o ? *o : def;
+        check_opts(opt);
+        memset(out, 0, sizeof(*out));
+        beep::count_noalloc(*pp->pp, *p->p, opt, stream, *out);
         if (out->stopped) throw Error(BEEP_ETIMEDOUT, "deadline passed: the counts are partial");
     });
 }

@@ -131,6 +131,11 @@ struct LaunchCfg {
     uint64_t deadline_ticks_lds = 0;  // LDS kernels
 };
 
+// Zero fill kernel (no memset nodes, so sequences can be captured) and the
+// kernel that arms a launch sequence's deadlines.
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
+hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st);
+
 int device_cus(int device);
 uint64_t wall_clock_khz(int device);
 
@@ -146,6 +151,8 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
 void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane,
                    unsigned long long *host_ctr);
 void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t st);
+// LDS lane classes, one launch each, round-robin over nst streams.
+void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst);
 void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res);
 // Balance figures from the busy-time accumulators (kProfPhases x kProfSlots).
 void fill_profile(const unsigned long long *prof, beep_result &res);

@@ -24,8 +24,6 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
                               uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
                               int *grid_out);
-hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
-hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
@@ -492,8 +490,7 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
 }
 
 // LDS lane classes, one launch each, round-robin over the given streams.
-static void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams,
-                                int nst)
+void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst)
 {
     if (c.kp.L < 3) return;
     const DevRoots &R = pp.dev->roots;

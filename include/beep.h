@@ -213,6 +213,25 @@ void beep_opts_default(beep_opts *o);
 /* Returns BEEP_ETIMEDOUT (with the partial result filled in) when the deadline passed. */
 int beep_count(beep_prep *pp, const beep_plan *p, const beep_opts *o, beep_result *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code running under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc: beep_count with every scratch buffer of the launch
+ * sequence (counters, work cursors, hub bitmap arena and hash, DFS stacks,
+ * donation ring, tuple buffer) carved from one fixed device array of
+ * BEEP_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends.  It calls no allocator.  pp must already be
+ * resident on o->device (beep_prep_upload).  The work runs on `stream` (a
+ * hipStream_t; NULL: the null stream), LDS batches before hub rounds, and the
+ * call waits for it; t_enum_ms is host wall time, t_hub_ms / t_lds_ms are 0.
+ * A query whose counters, hub arena, stacks and tuple buffer exceed the array
+ * fails with BEEP_ERANGE; the donation ring gets what is left (a hub item
+ * whose subtree finds the ring full is walked where it is).  opts.profile is
+ * refused (BEEP_EINVAL).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU race and must be serialised by the caller. */
+#define BEEP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int beep_count_noalloc(beep_prep *pp, const beep_plan *p, const beep_opts *o, void *stream,
+                       beep_result *out);
+
 /* ------------------------------------------------------------------------ */
 /* Batches.  No counterpart in the reference (one query per process there):  */
 /* several prepared queries over resident graphs run as one captured         */
