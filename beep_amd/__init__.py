@@ -144,28 +144,42 @@ class Plan:
 class Prepared:
     """Peeled + degree-oriented graph with the root queue, resident in HBM once uploaded."""
 
-    def __init__(self, graph, plan):
-        h = C.c_void_p()
-        check(lib().beep_prepare(graph._h, plan._h, C.byref(h)))
+    def __init__(self, graph, plan, _handle=None, _parent=None):
+        h = _handle
+        if h is None:
+            h = C.c_void_p()
+            check(lib().beep_prepare(graph._h, plan._h, C.byref(h)))
         self._h = h
+        self._parent = _parent  # keeps the shared variant's owner alive
         self.graph, self.plan = graph, plan
         d = self.describe()
         self.n, self.m, self.n_roots = d.n, d.m, d.n_roots
         self.alg_bytes = d.alg_bytes
+
+    def subset(self, ranks):
+        """The same query over the roots at the given (strictly ascending)
+        ranks of this one's root list."""
+        r = np.ascontiguousarray(np.asarray(ranks, dtype=np.uint64))
+        h = C.c_void_p()
+        check(lib().beep_prep_subset(self._h, self.plan._h, r.ctypes.data_as(_ffi.u64p), len(r), C.byref(h)))
+        return Prepared(self.graph, self.plan, _handle=h, _parent=self)
 
     def describe(self):
         d = _ffi.PrepDesc()
         check(lib().beep_prep_describe(self._h, C.byref(d)))
         return d
 
-    def arrays(self):
+    def arrays(self, copy=True):
+        """Host arrays of the prepared graph; with copy=False, views that are
+        valid while this object (and its graph variant) lives."""
         d = self.describe()
         n, m = d.n, d.m
 
         def A(ptr, k, dtype):  # host view of k entries (an empty vector may be NULL)
             if k == 0 or not ptr:
                 return np.zeros(0, dtype=dtype)
-            return np.ctypeslib.as_array(ptr, shape=(k,)).copy()
+            v = np.ctypeslib.as_array(ptr, shape=(k,))
+            return v.copy() if copy else v
 
         return dict(rowptr=A(d.rowptr, n + 1, np.uint64),
                     col=A(d.col, m, np.uint32),
@@ -176,7 +190,9 @@ class Prepared:
                     max_deg=d.max_degree, peel_rounds=d.peel_rounds, asc=d.asc,
                     n_hub_roots=d.n_hub_roots, n_lds_batches=d.n_lds_batches,
                     alg_bytes=d.alg_bytes, max_root_degree=d.max_root_degree,
-                    p1=bool(d.p1), p2=bool(d.p2), q0_head=d.q0_head, q0_head_larger=d.q0_head_larger)
+                    p1=bool(d.p1), p2=bool(d.p2), q0_head=d.q0_head, q0_head_larger=d.q0_head_larger,
+                    lds_class_roots=list(d.lds_class_roots), lds_class_units=list(d.lds_class_units),
+                    lds_class_bytes=list(d.lds_class_bytes))
 
     def preconditions(self):
         """P1 (no vertex above the root window) and P2 (the head skip drops
@@ -256,7 +272,8 @@ def _result_dict(r):
                 emitted=r.n_emitted, written=r.n_written, t_enum_ms=r.t_enum_ms,
                 t_hub_ms=r.t_hub_ms, t_lds_ms=r.t_lds_ms, donations=r.donations,
                 stopped=r.stopped, balance_lds=r.balance_lds, balance_hub=r.balance_hub,
-                donated_frac=r.donated_frac, lds_build_frac=r.lds_build_frac)
+                donated_frac=r.donated_frac, lds_build_frac=r.lds_build_frac,
+                t_lds_class_ms=list(r.t_lds_class_ms), balance_cu=r.balance_cu, cus_busy=r.cus_busy)
 
 
 class Batch:

@@ -3,7 +3,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbeep.so")
+# BEEP_LIB names another build of the library (e.g. for side-by-side timing).
+LIB_PATH = os.environ.get("BEEP_LIB") or os.path.join(HERE, "libbeep.so")
 MAXK = 10
 
 u32p = C.POINTER(C.c_uint32)
@@ -31,7 +32,9 @@ class PrepDesc(C.Structure):
                 ("peel_rounds", C.c_int), ("asc", C.c_int), ("alg_bytes", C.c_uint64),
                 ("n_hub_roots", C.c_uint64), ("n_lds_batches", C.c_uint64),
                 ("rowptr", u64p), ("col", u32p), ("ocol", u32p), ("nlow", u32p), ("roots", u32p),
-                ("p1", C.c_int), ("p2", C.c_int), ("q0_head", C.c_uint32), ("q0_head_larger", C.c_uint32)]
+                ("p1", C.c_int), ("p2", C.c_int), ("q0_head", C.c_uint32), ("q0_head_larger", C.c_uint32),
+                ("lds_class_roots", C.c_uint64 * 8), ("lds_class_units", C.c_uint64 * 8),
+                ("lds_class_bytes", C.c_uint64 * 8)]
 
 
 class Opts(C.Structure):
@@ -46,7 +49,8 @@ class Result(C.Structure):
                 ("n_emitted", C.c_uint64), ("n_written", C.c_uint64), ("t_enum_ms", C.c_double),
                 ("t_hub_ms", C.c_double), ("t_lds_ms", C.c_double), ("donations", C.c_uint64),
                 ("stopped", C.c_uint64), ("balance_lds", C.c_double), ("balance_hub", C.c_double),
-                ("donated_frac", C.c_double), ("lds_build_frac", C.c_double)]
+                ("donated_frac", C.c_double), ("lds_build_frac", C.c_double),
+                ("t_lds_class_ms", C.c_double * 8), ("balance_cu", C.c_double), ("cus_busy", C.c_uint64)]
 
 
 class ServerOpts(C.Structure):
@@ -85,6 +89,7 @@ SIGNATURES = {
     "beep_plan_free": (None, [C.c_void_p]),
     "beep_prepare": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "beep_prep_describe": (C.c_int, [C.c_void_p, C.POINTER(PrepDesc)]),
+    "beep_prep_subset": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "beep_prep_upload": (C.c_int, [C.c_void_p, C.c_int]),
     "beep_prep_free": (None, [C.c_void_p]),
     "beep_opts_default": (None, [C.POINTER(Opts)]),

@@ -280,6 +280,20 @@ int beep_prep_describe(const beep_prep *pp, beep_prep_desc *o)
         o->p2 = (!V.asc || P.boundary == 0 || P.q0_head_larger == 0) ? 1 : 0;
         o->q0_head = P.q0_head;
         o->q0_head_larger = P.q0_head_larger;
+        for (int c = 0; c < 8; c++) {
+            o->lds_class_roots[c] = P.cls_roots[c];
+            o->lds_class_units[c] = P.cls_units[c];
+            o->lds_class_bytes[c] = P.cls_bytes[c];
+        }
+    });
+}
+
+int beep_prep_subset(const beep_prep *pp, const beep_plan *p, const uint64_t *ranks, uint64_t n,
+                     beep_prep **out)
+{
+    return guarded([&] {
+        if (!pp || !p || !out || (n && !ranks)) throw Error(BEEP_EINVAL, "null argument");
+        *out = new beep_prep{beep::prepare_subset(*pp->pp, *p->p, ranks, n)};
     });
 }
 

@@ -100,6 +100,11 @@ struct Lane {
     hipStream_t side[kMaxSide] = {};
     hipEvent_t ev[4] = {};  // start, hubs done, all done, counters copied
     hipEvent_t side_done[kMaxSide] = {};
+    // end of each LDS lane class's launch (class c runs on side[c % nside])
+    static constexpr int kMaxClasses = 32;
+    hipEvent_t cls_done[kMaxClasses] = {};
+    uint8_t cls_lg[kMaxClasses] = {};
+    int ncls = 0;
     void create(int device, int nside);
     ~Lane();
 };
@@ -151,8 +156,13 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
 void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane,
                    unsigned long long *host_ctr);
 void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t st);
-// LDS lane classes, one launch each, round-robin over nst streams.
-void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst);
+// LDS lane classes, one launch each, round-robin over nst streams; with
+// cls_done, the end of class i's launch is recorded in cls_done[i] and its
+// lane class in cls_lg[i].  Returns the number of classes.
+int enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst,
+                        hipEvent_t *cls_done = nullptr, uint8_t *cls_lg = nullptr);
+// Device time of each lane class's launch (ms, by lane class) from a lane's events.
+void class_times(const Lane &lane, double *t_by_lg);
 void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res);
 // Balance figures from the busy-time accumulators (kProfPhases x kProfSlots).
 void fill_profile(const unsigned long long *prof, beep_result &res);

@@ -167,6 +167,11 @@ struct Prep {
     uint32_t n_hub = 0;
     std::vector<LdsBatch> batches;
     std::vector<uint8_t> batch_lanes_log2;  // per batch: log2 of DFS-group lanes
+    // Per lane class lg (one lds_batch_kernel launch each): roots, bitmap-build
+    // work units (range entries streamed, or root neighbours looked up) and
+    // the bytes the launch must read at least once (root records and rows,
+    // the row vertices' records and scan ranges, batch arrays).
+    uint64_t cls_roots[8] = {}, cls_units[8] = {}, cls_bytes[8] = {};
 
     DevPrep *dev = nullptr;
     ~Prep();
@@ -179,6 +184,10 @@ std::unique_ptr<Prep> prepare(const Graph &g, const Plan &p);
 // list (ordered by row length): every sample_stride-th LDS root and every
 // hub_sample_stride()-th hub root.
 std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &v, const Plan &p, uint32_t sample_stride = 1);
+// A prepared query over the roots of src at the given ranks of src.roots
+// (strictly ascending), sharing src's variant.
+std::unique_ptr<Prep> prepare_subset(const Prep &src, const Plan &p, const uint64_t *ranks, uint64_t n);
+void build_work(Prep &pp, const Plan &p, uint32_t sample_stride);
 // Stride at which a sampled root queue keeps hub roots (finer: hubs are few and heavy).
 inline uint32_t hub_sample_stride(uint32_t sample_stride) { return sample_stride >= 8 ? sample_stride / 8 : 1; }
 
@@ -200,5 +209,8 @@ struct LdsBudget {
 
 uint32_t hash_slots_for(uint32_t d);
 void build_tiers(Prep &pp);
+void class_figures(Prep &pp);
+// Bytes of the kernels' per-root record of an LDS batch (RootRec).
+constexpr uint64_t kRootRecBytes = 32;
 
 }  // namespace beep

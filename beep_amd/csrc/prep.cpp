@@ -213,7 +213,42 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
         pp->q0_head = h;
         pp->q0_head_larger = deg(h) - V.nlow[h];
     }
-    pp->max_root_degree = pp->roots.empty() ? 0 : deg(pp->roots[0]);
+    build_work(*pp, p, sample_stride);
+    return pp;
+}
+
+std::unique_ptr<Prep> prepare_subset(const Prep &src, const Plan &p, const uint64_t *ranks, uint64_t n)
+{
+    const Variant &V = *src.var;
+    if (V.min_qdeg != p.min_qdeg || V.asc != p.asc || src.max_qdeg != p.max_qdeg)
+        throw Error(BEEP_EINVAL, "plan does not match the prepared query");
+    auto pp = std::make_unique<Prep>();
+    pp->var = src.var;
+    pp->max_qdeg = src.max_qdeg;
+    pp->q0_size = src.q0_size;
+    pp->boundary = src.boundary;
+    pp->q0_head = src.q0_head;
+    pp->q0_head_larger = src.q0_head_larger;
+    pp->roots.resize(n);
+    for (uint64_t i = 0; i < n; i++) {
+        if (ranks[i] >= src.roots.size()) throw Error(BEEP_ERANGE, "root rank out of range");
+        if (i && ranks[i] <= ranks[i - 1]) throw Error(BEEP_EINVAL, "root ranks must be strictly ascending");
+        pp->roots[i] = src.roots[ranks[i]];
+    }
+    build_work(*pp, p, 1);
+    return pp;
+}
+
+// Root-dependent parts of a prepared query: B_alg, the work list (with a
+// stratified sample when sample_stride > 1) and its LDS / hub tiers.
+void build_work(Prep &pp_, const Plan &p, uint32_t sample_stride)
+{
+    Prep *pp = &pp_;
+    const Variant &V = *pp->var;
+    const auto &rp = V.rowptr;
+    auto deg = [&](uint32_t v) { return (uint32_t)(rp[v + 1] - rp[v]); };
+    pp->max_root_degree = 0;
+    for (uint32_t u : pp->roots) pp->max_root_degree = std::max(pp->max_root_degree, deg(u));
     // B_alg = sum over roots of 4(d(u)+2) + sum over neighbours 4(d(v)+2).
     std::atomic<uint64_t> bytes{0};
     parallel_for(pp->roots.size(), [&](size_t b, size_t e) {
@@ -278,7 +313,6 @@ std::unique_ptr<Prep> prepare_on(const std::shared_ptr<Variant> &vp, const Plan 
     }
     build_tiers(*pp);
     progress("root queue: %zu roots, %u hubs, %zu LDS batches", pp->roots.size(), pp->n_hub, pp->batches.size());
-    return pp;
 }
 
 // Roots whose d x ceil(d/64) bitmap fits the batch kernel's LDS budget are
@@ -329,6 +363,52 @@ void build_tiers(Prep &pp)
         }
         pp.batches.push_back(b);
         pp.batch_lanes_log2.push_back((uint8_t)lg);
+    }
+    class_figures(pp);
+}
+
+// Work and compulsory bytes of each lane class's launch, with the automatic
+// build mode (a row vertex whose scan range is over 32x the root's degree
+// looks the root's neighbours up by binary search, reading one entry and one
+// degree per step).
+void class_figures(Prep &pp)
+{
+    const Variant &V = *pp.var;
+    for (int c = 0; c < 8; c++) pp.cls_roots[c] = pp.cls_units[c] = pp.cls_bytes[c] = 0;
+    std::vector<uint64_t> bu(pp.batches.size()), bb(pp.batches.size());
+    parallel_for(pp.batches.size(), [&](size_t b0, size_t b1) {
+    for (size_t bi = b0; bi < b1; bi++) {
+        const LdsBatch &b = pp.batches[bi];
+        uint64_t units = 0, bytes = 8;  // batch first / count
+        for (uint32_t r = b.first; r < b.first + b.count; r++) {
+            const uint32_t u = pp.wu[r], d = pp.wlen[r];
+            bytes += kRootRecBytes + 4ull * d;
+            if (!pp.uses_bitmap) continue;
+            const uint64_t rb = V.rowptr[u] + pp.woff[r];
+            for (uint32_t j = 0; j < d; j++) {
+                const uint32_t v = V.ocol[rb + j];
+                const uint64_t hl = V.hlen[v];
+                bytes += 16;  // the row vertex's record
+                if (hl > 32ull * d) {
+                    uint64_t steps = 0;
+                    while ((1ull << steps) <= hl) steps++;
+                    units += d;
+                    bytes += 8ull * d * steps;
+                } else {
+                    units += hl;
+                    bytes += 4ull * hl;
+                }
+            }
+        }
+        bu[bi] = units;
+        bb[bi] = bytes;
+    }
+    }, 256);
+    for (size_t bi = 0; bi < pp.batches.size(); bi++) {
+        const int lg = pp.batch_lanes_log2[bi];
+        pp.cls_roots[lg] += pp.batches[bi].count;
+        pp.cls_units[lg] += bu[bi];
+        pp.cls_bytes[lg] += bb[bi];
     }
 }
 

@@ -20,6 +20,8 @@
 
 namespace beep {
 
+static_assert(sizeof(RootRec) == kRootRecBytes, "per-root record size");
+
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
                               uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
@@ -83,6 +85,7 @@ void Lane::create(int dev, int ns)
     for (int i = 0; i < nside; i++) HIP_CHECK(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking));
     for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
     for (int i = 0; i < nside; i++) HIP_CHECK(hipEventCreate(&side_done[i]));
+    for (auto &e : cls_done) HIP_CHECK(hipEventCreate(&e));
 }
 
 Lane::~Lane()
@@ -91,6 +94,8 @@ Lane::~Lane()
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : side_done)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : cls_done)
         if (e) (void)hipEventDestroy(e);
     for (auto &s : side)
         if (s) (void)hipStreamDestroy(s);
@@ -354,12 +359,26 @@ static bool tail_summed(const Plan &p, int emit)
     return emit == BEEP_EMIT_NONE && p.L >= 4 && !((p.qmask[p.L - 1] >> (p.L - 2)) & 1u);
 }
 
+// Count-only plans whose last three levels pair up (KPlan::pair).
+static bool pair_summed(const Plan &p, int emit)
+{
+    if (emit != BEEP_EMIT_NONE || p.L < 5) return false;
+    const int a = p.L - 3, b = p.L - 2, c = p.L - 1;
+    const uint32_t below_a = (1u << a) - 1u;
+    if (p.qmask[c] != (uint32_t)(p.qmask[a] | (1u << b))) return false;
+    if (!((p.qmask[b] >> a) & 1u) || (p.qmask[b] & ~(1u << a) & ~below_a)) return false;
+    if (p.smask[c] != (uint32_t)(p.smask[a] | (1u << a))) return false;
+    return (p.smask[b] & ~below_a) == 0;
+}
+
 // Hub items donate level-2 subtrees; not when everything below x_1 is one
-// summed level (L = 4 with a summed tail): an item then costs one pass over
-// its d/64 words whatever |C_2| is.
+// summed level (L = 4 with a summed tail) or one summed pair of levels (L = 5
+// with paired levels): an item then costs one or |B| passes over its d/64
+// words whatever |C_2| is.
 static bool hub_donates(const Plan &p, const beep_opts &o, const DevRoots &R)
 {
-    return o.donate && R.nhubs && p.L >= 4 && !(p.L == 4 && tail_summed(p, o.emit));
+    return o.donate && R.nhubs && p.L >= 4 && !(p.L == 4 && tail_summed(p, o.emit)) &&
+           !(p.L == 5 && pair_summed(p, o.emit));
 }
 
 static int hub_grid_of(const DevRoots &R, int nlev, int cus)
@@ -399,6 +418,7 @@ static KPlan make_kplan(const Plan &p, int emit, int build_mode)
     k.emit = emit;
     k.bin_ratio = build_mode == BEEP_BUILD_SCAN ? ~0u : build_mode == BEEP_BUILD_SEARCH ? 0u : 32u;
     k.tail = tail_summed(p, emit) ? 1 : 0;
+    k.pair = pair_summed(p, emit) ? 1 : 0;
     for (int l = 0; l < kMaxLevels; l++) {
         k.qmask[l] = p.qmask[l];
         k.smask[l] = p.smask[l];
@@ -490,9 +510,10 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
 }
 
 // LDS lane classes, one launch each, round-robin over the given streams.
-void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst)
+int enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst,
+                        hipEvent_t *cls_done, uint8_t *cls_lg)
 {
-    if (c.kp.L < 3) return;
+    if (c.kp.L < 3) return 0;
     const DevRoots &R = pp.dev->roots;
     uint32_t b = 0;
     int cls = 0;
@@ -504,8 +525,25 @@ void enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, h
         unsigned long long *cur = s.ctr + kCtrLds + (uint64_t)cls * kCurWords;
         HIP_CHECK(launch_lds_batches(lg, pp.dev->cus, streams[cls % nst], c.kg, c.kp, R.recs, R.batch_first,
                                      R.batch_count, b, e, cur, c.out, nullptr));
+        if (cls_done) {
+            HIP_CHECK(hipEventRecord(cls_done[cls], streams[cls % nst]));
+            cls_lg[cls] = (uint8_t)lg;
+        }
         b = e;
         cls++;
+    }
+    return cls;
+}
+
+void class_times(const Lane &lane, double *t_by_lg)
+{
+    for (int i = 0; i < 8; i++) t_by_lg[i] = 0;
+    for (int i = 0; i < lane.ncls; i++) {
+        // a class starts when the class before it on its stream ends (or at ev[0])
+        const hipEvent_t start = i >= lane.nside ? lane.cls_done[i - lane.nside] : lane.ev[0];
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, start, lane.cls_done[i]));
+        if (lane.cls_lg[i] < 8) t_by_lg[lane.cls_lg[i]] += t;
     }
 }
 
@@ -519,7 +557,7 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
     // The LDS lane classes run on the side streams, concurrently with the hub
     // rounds on the main stream.
     for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipStreamWaitEvent(lane.side[i], lane.ev[0], 0));
-    enqueue_lds_classes(pp, c, s, lane.side, lane.nside);
+    lane.ncls = enqueue_lds_classes(pp, c, s, lane.side, lane.nside, lane.cls_done, lane.cls_lg);
     for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipEventRecord(lane.side_done[i], lane.side[i]));
     enqueue_hubs(pp, c, s, st);
     HIP_CHECK(hipEventRecord(lane.ev[1], st));
@@ -574,6 +612,14 @@ void fill_profile(const unsigned long long *prof, beep_result &res)
     }
     res.balance_lds = balance(lds);
     res.balance_hub = balance(hub);
+    std::vector<double> cu(kProfSlots);
+    uint64_t busy = 0;
+    for (int i = 0; i < kProfSlots; i++) {
+        cu[i] = (double)prof[kProfCuPhase * kProfSlots + i];
+        busy += prof[kProfCuPhase * kProfSlots + i] ? 1 : 0;
+    }
+    res.balance_cu = balance(cu);
+    res.cus_busy = busy;
     res.donated_frac = (own + don) > 0 ? don / (own + don) : 0.0;
     res.lds_build_frac = lds_sum > 0 ? build / lds_sum : 0.0;
 }
@@ -615,6 +661,7 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     res.t_enum_ms = t_all;
     res.t_hub_ms = t_hub;
     res.t_lds_ms = t_lds;
+    class_times(d.lane, res.t_lds_class_ms);
     if (c.out.buf && o.match_buf && res.n_written)
         HIP_CHECK(hipMemcpy(o.match_buf, c.out.buf, res.n_written * (uint64_t)p.L * sizeof(uint32_t),
                             hipMemcpyDeviceToHost));

@@ -148,11 +148,25 @@ __device__ __forceinline__ void mark_stop(const KOut &out) { atomicOr(out.stop, 
 // DFS steps between two deadline checks.
 constexpr uint32_t kClockEvery = 1024;
 
+// Compute unit the wave runs on, as a slot below 2048: the XCC (HW_REG_XCC_ID
+// bits 2:0) and, within it, shader engine, shader array and CU (HW_REG_HW_ID
+// bits 15:8).  The reference reads %smid for its per-SM times (utils.cuh:66-99).
+__device__ __forceinline__ uint32_t cu_slot()
+{
+    const uint32_t hw = __builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 4);    // HW_ID, 32 bits
+    const uint32_t xcc = __builtin_amdgcn_s_getreg((15 << 11) | (0 << 6) | 20);  // XCC_ID, 16 bits
+    return ((xcc & 7u) << 8) | ((hw >> 8) & 0xffu);
+}
+
+// Busy time since t0 into phase `phase`, slot `slot`; phases 0-2 (work, not
+// the build share) also into the compute unit's slot of phase kProfCuPhase.
 __device__ __forceinline__ void prof_add(const KOut &out, int phase, uint32_t slot, uint64_t t0)
 {
-    if (out.prof)
-        atomicAdd(&out.prof[phase * kProfSlots + (slot % kProfSlots)],
-                  (unsigned long long)((uint64_t)wall_clock64() - t0));
+    if (out.prof) {
+        const unsigned long long dt = (unsigned long long)((uint64_t)wall_clock64() - t0);
+        atomicAdd(&out.prof[phase * kProfSlots + (slot % kProfSlots)], dt);
+        if (phase < 3) atomicAdd(&out.prof[kProfCuPhase * kProfSlots + cu_slot()], dt);
+    }
 }
 
 // Flushes per-lane accumulators: one atomic per wave and counter, into the
@@ -619,6 +633,44 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             return (uint64_t)sc * ftot - (uint64_t)__popcll(S & F);
         };
 
+        // Paired last levels (pl.pair), l = L-2 = b: S = C_{l-1} = C_a with
+        // x_1..x_{a-1} fixed; adds C(|S n E[x_b]|, 2) over x_b in B, the
+        // candidates of level b without its constraints on x_a.
+        auto pair_sum = [&](int l) -> uint64_t {
+            const uint64_t S = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
+            uint64_t B = 0;
+            if (own) {
+                B = valid_mask(w, d);
+                uint32_t qm = sMisc[M_QM + l] & ~(1u << (l - 1));
+                while (qm) {
+                    const int q = __builtin_ctz(qm);
+                    qm &= qm - 1;
+                    B &= E[xget(q) * W + w];
+                }
+                for (int s = 1; s < l - 1; s++) B &= ~bit_at(xget(s), w);
+                uint32_t sm = sMisc[M_SM + l];
+                while (sm) {
+                    const int s = __builtin_ctz(sm);
+                    sm &= sm - 1;
+                    B &= gt_mask(xget(s), w);
+                }
+            }
+            uint64_t sum = 0;
+            for (;;) {
+                const uint64_t bal = __ballot(B != 0ull);
+                const uint64_t mine = (P == 64) ? bal : ((bal >> pbase) & ((1ull << P) - 1ull));
+                if (mine == 0ull) break;
+                const int f = __builtin_ctzll(mine);
+                const uint64_t vf = __shfl(B, f, P);
+                const uint32_t xb = ((uint32_t)f << 6) + (uint32_t)__builtin_ctzll(vf);
+                if (lx == f) B &= B - 1ull;  // lane f's lowest bit is x_b
+                uint32_t pc = own ? (uint32_t)__popcll(S & E[xb * W + w]) : 0u;
+                for (int o = 1; o < P; o <<= 1) pc += __shfl_xor(pc, o, P);
+                if (lx == 0) sum += (uint64_t)pc * (pc - 1u) / 2u;
+            }
+            return sum;
+        };
+
         const uint64_t t2 = own ? level_word(2) : 0ull;
         if (L == 3) {
             cnt += (uint64_t)__popcll(t2);
@@ -635,6 +687,13 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
             }
             if (pl.tail && l == L - 1) {  // every x_{L-2} of C_{L-2} at once
                 cnt += tail_sum(l);
+                if (l == 3) break;
+                --l;
+                pos = (uint32_t)xget(l - 1) + 1u;
+                continue;
+            }
+            if (pl.pair && l == L - 2) {  // levels L-3..L-1 at once
+                cnt += pair_sum(l);
                 if (l == 3) break;
                 --l;
                 pos = (uint32_t)xget(l - 1) + 1u;
@@ -1069,6 +1128,68 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         cnt -= acc;  // the wave's sum is exact modulo 2^64
     };
 
+    // Paired last levels (pl.pair), l = L-2 = b: S = C_{l-1} in the stack with
+    // x_1..x_{l-2} fixed; B (level b without its constraints on x_a = x_{l-1})
+    // goes to the next stack row, then C(|S n E[x_b]|, 2) is added for every
+    // x_b in B.
+    auto pair_count = [&](int l) {
+        const uint64_t *S = my + (uint64_t)(l - 3) * wmax;
+        uint64_t *Bv = my + (uint64_t)(l - 2) * wmax;
+        const uint32_t blo = word_lo(l);
+        const uint32_t slo = sWlo[wv][l - 1];
+        for (uint32_t rr = 0; rr < R; rr++) {
+            const uint32_t w = rr * 64 + lane;
+            if (w >= W) break;
+            uint64_t t = 0;
+            if (w >= blo) {
+                t = valid_mask(w, d);
+                uint32_t qm = sQM[l] & ~(1u << (l - 1));
+                while (qm) {
+                    const int q = __builtin_ctz(qm);
+                    qm &= qm - 1;
+                    if (q == 1 && row1) t &= e1[w];
+                    else t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
+                }
+                for (int s = 1; s < l - 1; s++) t &= ~bit_at(xget(s), w);
+                uint32_t sm = sSM[l];
+                while (sm) {
+                    const int s = __builtin_ctz(sm);
+                    sm &= sm - 1;
+                    t &= gt_mask(xget(s), w);
+                }
+            }
+            Bv[w] = t;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t pos = blo << 6;
+        for (;;) {
+            uint32_t xb = 0xffffffffu;
+            const uint32_t pw = pos >> 6;
+            for (uint32_t rr = pw >> 6; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                uint64_t v = (w < W && w >= pw) ? Bv[w] : 0ull;
+                if (w == pw) v &= (~0ull << (pos & 63u));
+                const uint64_t bal = __ballot(v != 0ull);
+                if (bal) {
+                    const int f = __builtin_ctzll(bal);
+                    const uint64_t vf = __shfl(v, f, 64);
+                    xb = ((rr * 64 + (uint32_t)f) << 6) + (uint32_t)__builtin_ctzll(vf);
+                    break;
+                }
+            }
+            if (xb == 0xffffffffu) break;
+            const uint64_t *Eb = Eh + (uint64_t)(xb - rlo) * W;
+            uint32_t pc = 0;
+            for (uint32_t rr = slo >> 6; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                if (w < W && w >= slo) pc += (uint32_t)__popcll(S[w] & Eb[w]);
+            }
+            pc = (uint32_t)wave_sum_u64(pc);
+            if (lane == 0) cnt += (uint64_t)pc * (pc - 1u) / 2u;
+            pos = xb + 1u;
+        }
+    };
+
     // DFS below level 2 with C_2 in my[0, W): x_2 runs over C_2 n [pos0, hi2).
     auto dfs3 = [&](uint32_t pos0, uint32_t hi2) {
         int l = 3;
@@ -1080,6 +1201,13 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             }
             if (pl.tail && l == L - 1) {  // every x_{L-2} of the level at once
                 tail_count(l, pos, l == 3 ? hi2 : 0xffffffffu);
+                if (l == 3) break;
+                --l;
+                pos = xget(l - 1) + 1u;
+                continue;
+            }
+            if (pl.pair && l == L - 2) {  // levels L-3..L-1 at once (never split by donation at l = 3)
+                pair_count(l);
                 if (l == 3) break;
                 --l;
                 pos = xget(l - 1) + 1u;

@@ -37,6 +37,15 @@ struct KPlan {
     // The kernels then sum |C_{L-1}| over all x_{L-2} in C_{L-2} in one pass
     // (prefix popcounts) instead of walking x_{L-2}.
     int32_t tail;
+    // Count-only plans whose last three levels a = L-3, b = L-2, c = L-1 have
+    //   qmask[c] = qmask[a] + {b},  smask[c] = smask[a] + {a},
+    //   a in qmask[b], and qmask[b] - {a}, smask[b] below a:
+    // x_c ranges over the same set S = C_a as x_a, restricted to the row of
+    // x_b and above x_a, and x_b does not depend on x_a otherwise.  For fixed
+    // x_1..x_{a-1}, the levels a..c then add up to
+    //   sum over x_b in B of C(|S n E[x_b]|, 2),
+    // B = the candidates of b without the a constraints (hub + 4-cycle, W5).
+    int32_t pair;
     uint32_t qmask[kMaxLevels];
     uint32_t smask[kMaxLevels];
 };
@@ -58,6 +67,9 @@ struct KGraph {
 
 // Staged range of a row vertex: start (40 bits) and length (24 bits).
 constexpr int kRangeShift = 40;
+// LDS-batch row record: range start (bits 0-39) and flags -- the row looks the
+// root's neighbours up by binary search, or it has no work (one idle unit).
+constexpr int kRowSearch = 62, kRowIdle = 63;
 
 // Per-root record of an LDS batch, offsets relative to the batch.
 struct RootRec {
@@ -127,11 +139,12 @@ struct KOut {
     // Optional busy-time accumulators (wall_clock64 ticks), kProfPhases x
     // kProfSlots: phase 0 LDS workgroups, 1 hub waves on their own items, 2 hub
     // waves on donated subtrees, 3 the LDS workgroups' staging and bitmap
-    // build; slot = workgroup (LDS) or wave (hub) index.
+    // build; slot = workgroup (LDS) or wave (hub) index.  Phase 4 sums
+    // phases 0-2 by compute unit (slot = XCC, shader engine, array, CU).
     unsigned long long *prof = nullptr;
 };
 
-constexpr int kProfSlots = 4096, kProfPhases = 4;
+constexpr int kProfSlots = 4096, kProfPhases = 5, kProfCuPhase = 4;
 
 // One prepared query of a multi-query launch.
 struct MultiQuery {

@@ -8,21 +8,24 @@ sequence in sgm mode (counting, no tuple output -- the reference's sgm).
 
 Reported:
   value        matches/s = (count_4clique + count_diamond) / device time
-  roofline     B_alg / t_enum against 8 TB/s, B_alg = sum over roots of
-               4(d(u)+2) + sum over neighbours 4(d(v)+2) (peeled degrees);
-               t_enum is the HIP-event time of the query's launch sequence on
-               the library's stream.  B_alg charges every root with the whole
-               row of every neighbour; the kernels read only the larger-key
-               part of a neighbour's row, sum some last levels in closed form
-               and have re-read rows served by L2 / Infinity Cache, so
-               B_alg / t_enum can exceed the HBM peak.  "traffic" is the
-               measured HBM bytes per step (committed rocprofv3 FETCH_SIZE /
-               WRITE_SIZE summary) and "measured_gbs" / "measured_frac" the
-               HBM rate it gives over the same device time.
+  roofline     the dominant kernel, lds_batch_kernel<1> (LDS batches of roots
+               with at most 64 row entries): the bytes its launches must read
+               at least once (root records and rows, the row vertices' records
+               and bitmap-build scan ranges; computed on the host per lane
+               class) over its device time, measured with HIP events on the
+               stream it runs on, against the 8 TB/s HBM peak.  "traffic" is
+               its measured HBM bytes per step (committed rocprofv3
+               FETCH_SIZE / WRITE_SIZE summary); "b_alg_per_step" is
+               sum over roots of 4(d(u)+2) + sum over neighbours 4(d(v)+2)
+               (peeled degrees), which charges every root with the whole row of
+               every neighbour and is not what any kernel reads.
   cpu_baseline the oracle (CPU restatement of the reference enumeration,
-               OpenMP over roots) on the same prepared graphs, all host cores
-               up to 16, same two queries
-  parity       GPU counts and tuple-multiset hashes equal to the oracle's
+               OpenMP over roots, count only, no summed levels) on the same
+               prepared graphs, same two queries, with the host's CPU model,
+               its logical CPU count and the threads used
+  parity       GPU counts and tuple-multiset hashes (count-only and emitting
+               runs) equal to the oracle's
+  emit_hash    the same step with every sgm match walked and hashed
 
 Other modes (each prints one JSON line of its own):
   --mode batched  rmat22cut: the 8 named central-node patterns (sgc), one
@@ -92,15 +95,46 @@ def calib_copy(mib):
 
 
 def read_traffic(workload):
-    """Bytes per step from the committed FETCH_SIZE/WRITE_SIZE summary, if any."""
+    """HBM bytes from the committed FETCH_SIZE/WRITE_SIZE summary, if any:
+    per step and per launch of the dominant kernel."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     with open(path) as f:
         for ln in f:
-            if ln.startswith("bytes_per_step"):
-                return float(ln.split("=")[1])
-    return None
+            if "=" in ln and not ln.startswith("#"):
+                k, v = ln.split("=", 1)
+                try:
+                    out[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    return out
+
+
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+# The kernel the roofline line is about: lds_batch_kernel<1>, the LDS batches of
+# roots with at most 64 row entries (lane class 0, one DFS lane per item) -- the
+# largest share of device time in the kernel trace of this workload.
+DOMINANT_LG = 0
+DOMINANT_KERNEL = "lds_batch_kernel<1>"
 
 
 def main():
@@ -140,11 +174,14 @@ def main():
     for _, _, _, pp in preps:
         pp.upload(0)
     t_upload = time.time() - t0
+    descs = [pp.describe() for _, _, _, pp in preps]
 
     if args.once:
-        for name, task, plan, pp in preps:
+        for (name, task, plan, pp), dsc in zip(preps, descs):
             r = pp.count(emit=B.EMIT_NONE, lanes=args.lanes)
             print(json.dumps({"query": name, "count": r["count"], "t_enum_ms": r["t_enum_ms"],
+                              "t_lds_class_ms": r["t_lds_class_ms"],
+                              "lds_class_bytes": list(dsc.lds_class_bytes),
                               "alg_bytes": pp.alg_bytes, "donations": r["donations"]}))
         if args.calib_copy:
             calib_copy(args.calib_copy)
@@ -156,44 +193,60 @@ def main():
         r = pp.count(emit=B.EMIT_HASH, lanes=args.lanes)
         gpu_ref.append(r)
 
-    for _ in range(args.warmup):
-        for _, _, _, pp in preps:
-            pp.count(emit=B.EMIT_NONE, lanes=args.lanes)
-    per_query_ms = [0.0] * len(preps)
-    per_query_hub = [0.0] * len(preps)
-    per_query_lds = [0.0] * len(preps)
-    counts = [0] * len(preps)
-    wall0 = time.time()
-    for _ in range(args.steps):
-        for i, (_, _, _, pp) in enumerate(preps):
-            r = pp.count(emit=B.EMIT_NONE, lanes=args.lanes)
-            per_query_ms[i] += r["t_enum_ms"]
-            per_query_hub[i] += r["t_hub_ms"]
-            per_query_lds[i] += r["t_lds_ms"]
-            counts[i] = r["count"]
-    wall = time.time() - wall0
-    # one profiled run per query (untimed): busy-time balance of workgroups / waves
+    def timed(emit):
+        for _ in range(args.warmup):
+            for _, _, _, pp in preps:
+                pp.count(emit=emit, lanes=args.lanes)
+        n = len(preps)
+        tot, hub, lds, dom = [0.0] * n, [0.0] * n, [0.0] * n, [0.0] * n
+        cnt, hs = [0] * n, [None] * n
+        wall0 = time.time()
+        for _ in range(args.steps):
+            for i, (_, _, _, pp) in enumerate(preps):
+                r = pp.count(emit=emit, lanes=args.lanes)
+                tot[i] += r["t_enum_ms"]
+                hub[i] += r["t_hub_ms"]
+                lds[i] += r["t_lds_ms"]
+                dom[i] += r["t_lds_class_ms"][DOMINANT_LG]
+                cnt[i] = r["count"]
+                hs[i] = (r["hash_sum"], r["hash_xor"])
+        return dict(wall=time.time() - wall0, tot=tot, hub=hub, lds=lds, dom=dom, counts=cnt, hashes=hs)
+
+    plain = timed(B.EMIT_NONE)
+    emitting = timed(B.EMIT_HASH)
+    K = args.steps
+    # one profiled run per query (untimed): busy-time balance of workgroups,
+    # waves and compute units
     balance = {}
     for (q, _), (_, _, _, pp) in zip(queries, preps):
         r = pp.count(emit=B.EMIT_NONE, lanes=args.lanes, profile=True)
         balance[q] = {"lds": round(r["balance_lds"], 3), "hub": round(r["balance_hub"], 3),
+                      "cu": round(r["balance_cu"], 3), "cus_busy": r["cus_busy"],
                       "donated_frac": round(r["donated_frac"], 3),
                       "lds_build_frac": round(r["lds_build_frac"], 3)}
-    K = args.steps
-    dev_ms_step = sum(per_query_ms) / K
+    dev_ms_step = sum(plain["tot"]) / K
+    counts = plain["counts"]
     matches = sum(counts)
     value = matches / (dev_ms_step / 1000.0)
-    alg_bytes = sum(pp.alg_bytes for _, _, _, pp in preps)
-    achieved = alg_bytes / (dev_ms_step / 1000.0) / 1e9
+    b_alg = sum(pp.alg_bytes for _, _, _, pp in preps)
+    # dominant kernel: compulsory bytes of its launches / their device time
+    dom_bytes = sum(dsc.lds_class_bytes[DOMINANT_LG] for dsc in descs)
+    dom_ms = sum(plain["dom"]) / K
+    achieved = dom_bytes / (dom_ms / 1000.0) / 1e9 if dom_ms > 0 else None
     traffic = read_traffic(args.workload)
+    dom_traffic = traffic.get("dominant_bytes_per_step")
+    step_traffic = traffic.get("bytes_per_step")
 
-    # CPU baseline + parity (oracle, same prepared graphs)
+    # CPU baseline: the oracle (the reference's enumeration restated, OpenMP over
+    # roots, no summed levels) counting only, on the same prepared graphs; then,
+    # untimed, the same with tuple hashes for parity with the GPU's.
     cpu = None
     parity = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
-        cores = min(os.cpu_count() or 1, 16)
+        model, nproc, usable = host_cpu()
+        cores = min(usable, 16)
         rp, col = g.csr()
         src = np.repeat(np.arange(g.n, dtype=np.uint32), np.diff(rp).astype(np.int64))
         og = O.Graph(src, col)
@@ -204,17 +257,23 @@ def main():
             s, d = P.directed(P.read_mtx_edges(name))
             op = O.Prepared(og, O.plan(s, d, B.mode_of(task)))
             t0 = time.time()
-            o = op.count(nthreads=cores, want_hash=True)
+            o = op.count(nthreads=cores, want_hash=False)
             cpu_s += time.time() - t0
             cpu_matches += o["count"]
-            ok = (o["count"] == gr["count"] and o["hash_sum"] == gr["hash_sum"]
-                  and o["hash_xor"] == gr["hash_xor"])
+            oh = op.count(nthreads=cores, want_hash=True)
+            ok = (o["count"] == gr["count"] == oh["count"] and oh["hash_sum"] == gr["hash_sum"]
+                  and oh["hash_xor"] == gr["hash_xor"])
             parity = parity and ok
         cpu = {"value": cpu_matches / cpu_s, "unit": "matches/s", "cores": cores, "kind": "port",
+               "nproc": nproc, "cpus_usable": usable, "cpu_model": model,
                "sample": f"full workload ({' then '.join(q for q, _ in queries)}) on the same "
-                         f"prepared graphs, {cpu_s:.2f} s of enumeration, tuple hashes on"}
-        parity = bool(parity) and counts == [r["count"] for r in gpu_ref]
+                         f"prepared graphs, count only, {cpu_s:.2f} s of enumeration on {cores} "
+                         f"threads (the host grants 16 CPUs per GPU)"}
+        parity = (bool(parity) and counts == [r["count"] for r in gpu_ref]
+                  and emitting["counts"] == counts
+                  and emitting["hashes"] == [(r["hash_sum"], r["hash_xor"]) for r in gpu_ref])
 
+    emit_ms = sum(emitting["tot"]) / K
     out = {
         "metric": "subgraph matches/s",
         "value": value,
@@ -222,7 +281,7 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1000.0 / K,
+        "ms_per_step": plain["wall"] * 1000.0 / K,
         "device_ms_per_step": dev_ms_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -236,15 +295,26 @@ def main():
             "n": g.n, "directed_entries": g.m, "queries": [q for q, _ in queries],
             "counts": counts, "lanes": args.lanes or "auto",
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "alg_bytes_per_step": alg_bytes,
-                     "measured_gbs": traffic / (dev_ms_step / 1000.0) / 1e9 if traffic else None,
-                     "measured_frac": traffic / (dev_ms_step / 1000.0) / 1e9 / PEAK_HBM_GBS if traffic else None},
+        "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL,
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None,
+                     "traffic": dom_traffic,
+                     "alg_bytes_per_launch": dom_bytes / max(len(preps), 1),
+                     "launches_per_step": len(preps),
+                     "kernel_ms_per_step": dom_ms,
+                     "measured_gbs": dom_traffic / (dom_ms / 1000.0) / 1e9 if dom_traffic and dom_ms else None,
+                     "step_traffic": step_traffic,
+                     "step_measured_gbs": (step_traffic / (dev_ms_step / 1000.0) / 1e9
+                                           if step_traffic else None),
+                     "b_alg_per_step": b_alg},
         "cpu_baseline": cpu,
         "parity": parity,
-        "per_query_ms": {q: per_query_ms[i] / K for i, (q, _) in enumerate(queries)},
-        "per_query_phase_ms": {q: {"hub": per_query_hub[i] / K, "lds": per_query_lds[i] / K}
+        "emit_hash": {"value": sum(emitting["counts"]) / (emit_ms / 1000.0), "unit": "matches/s",
+                      "device_ms_per_step": emit_ms,
+                      "note": "sgm match sets: every tuple walked and hashed, no summed last level"},
+        "per_query_ms": {q: plain["tot"][i] / K for i, (q, _) in enumerate(queries)},
+        "per_query_phase_ms": {q: {"hub": plain["hub"][i] / K, "lds": plain["lds"][i] / K,
+                                   "dominant_kernel": plain["dom"][i] / K}
                                for i, (q, _) in enumerate(queries)},
         "balance": balance,
         "host_s": {"generate": round(t_gen, 3), "prepare": round(t_prep, 3),
@@ -541,14 +611,19 @@ def run_large(args):
             pp = B.Prepared(g, B.Plan.from_edges(s, d, B.mode_of(task)))
             pp.upload(0)
             rec["prepare_s"] = round(time.time() - t0, 2)
+            rec["preconditions"] = pp.preconditions()
+            dsc = pp.describe()
             r = pp.count(deadline_ms=dl, profile=True)
+            dom_ms = r["t_lds_class_ms"][DOMINANT_LG]
             rec.update(count=r["count"], t_enum_ms=r["t_enum_ms"], hub_ms=r["t_hub_ms"], lds_ms=r["t_lds_ms"],
-                       donations=r["donations"], stopped=r["stopped"], alg_bytes=pp.alg_bytes,
+                       donations=r["donations"], stopped=r["stopped"], b_alg=pp.alg_bytes,
                        balance_lds=r["balance_lds"], balance_hub=r["balance_hub"],
+                       balance_cu=r["balance_cu"], cus_busy=r["cus_busy"],
                        donated_frac=r["donated_frac"],
-                       achieved_gbs=pp.alg_bytes / (r["t_enum_ms"] / 1000.0) / 1e9,
+                       dominant_kernel_ms=dom_ms, dominant_kernel_bytes=dsc.lds_class_bytes[DOMINANT_LG],
                        matches_per_s=r["count"] / (r["t_enum_ms"] / 1000.0))
-            rec["frac"] = rec["achieved_gbs"] / PEAK_HBM_GBS
+            rec["dominant_kernel_gbs"] = (dsc.lds_class_bytes[DOMINANT_LG] / (dom_ms / 1000.0) / 1e9
+                                          if dom_ms > 0 else None)
             if not r["stopped"]:
                 ran.append(pp)
         except B.BeepError as e:
@@ -579,13 +654,16 @@ def run_large(args):
                                f"counts samples), seed {wl['seed']}, {g.m // 2} undirected edges after "
                                f"dedup, max degree {g.max_degree}",
                    "queries": rows, "deadline_ms": dl},
-        "roofline": {"bound": "hbm", "achieved": (sum(r["alg_bytes"] for r in done) / (tot_ms / 1000.0) / 1e9)
-                     if tot_ms else None, "peak": PEAK_HBM_GBS, "unit": "GB/s"},
+        "roofline": None,
         "batched": batched,
         "host_s": {"generate": round(t_gen, 1)},
     }
-    if out["roofline"]["achieved"] is not None:
-        out["roofline"]["frac"] = out["roofline"]["achieved"] / PEAK_HBM_GBS
+    dom = [r for r in rows if r.get("dominant_kernel_ms")]
+    dms = sum(r["dominant_kernel_ms"] for r in dom)
+    if dms > 0:
+        ach = sum(r["dominant_kernel_bytes"] for r in dom) / (dms / 1000.0) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": DOMINANT_KERNEL, "achieved": ach, "peak": PEAK_HBM_GBS,
+                           "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None}
     print(json.dumps(out))
 
 

@@ -160,9 +160,23 @@ typedef struct {
     int p1, p2;
     uint32_t q0_head;        /* Q0[0] (UINT32_MAX when Q0 is empty) */
     uint32_t q0_head_larger; /* neighbours of Q0[0] with a larger (degree, id) key */
+    /* Per lane class lg (LDS batches of roots with at most 64 << lg row
+     * entries; one lds_batch_kernel launch each): roots, bitmap-build work
+     * units (scan-range entries streamed, or root neighbours looked up) and
+     * the bytes the launch must read at least once (root records and rows,
+     * row-vertex records and scan ranges, batch arrays). */
+    uint64_t lds_class_roots[8], lds_class_units[8], lds_class_bytes[8];
 } beep_prep_desc;
 
 int beep_prep_describe(const beep_prep *pp, beep_prep_desc *out);
+/* The same query over a subset of pp's roots: ranks index beep_prep_desc.roots
+ * (R in processing order) and must be strictly ascending.  The result shares
+ * pp's peeled, oriented graph (and its HBM copy).  The reference has no such
+ * call: its launcher takes every root of R (subgraph_matching.cuh:1080-1131);
+ * its multi-GPU split hands each device a part of R the same way
+ * (sgm_kernelsLD.cuh:60, subgraph_matching.cuh:1167-1170). */
+int beep_prep_subset(const beep_prep *pp, const beep_plan *p, const uint64_t *ranks, uint64_t n,
+                     beep_prep **out);
 /* Copies the prepared graph to HBM on `device` (idempotent). */
 int beep_prep_upload(beep_prep *pp, int device);
 void beep_prep_free(beep_prep *pp);
@@ -207,6 +221,13 @@ typedef struct {
      * share of hub-DFS busy time spent on donated subtrees, and the share of
      * LDS workgroup time spent staging rows and building bitmaps */
     double balance_lds, balance_hub, donated_frac, lds_build_frac;
+    /* device time of each lane class's lds_batch_kernel launch (beep_count; 0: none) */
+    double t_lds_class_ms[8];
+    /* with opts.profile: mean / max busy time over the compute units that ran
+     * LDS workgroups or hub waves (the reference's per-SM utilisation,
+     * sgm_kernelsLD_timer.cuh:339-349), and how many compute units did */
+    double balance_cu;
+    uint64_t cus_busy;
 } beep_result;
 
 void beep_opts_default(beep_opts *o);

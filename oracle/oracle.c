@@ -729,14 +729,16 @@ static void enum_root(dfs_ctx *c, uint32_t u)
     }
 }
 
-static int run_roots(const or_prep *pp, const or_plan *p, uint64_t r0, uint64_t r1, int nthreads,
-                     int want_hash, uint32_t *match_buf, uint64_t match_cap, or_result *res)
+/* Roots roots[r] for r in [r0, r1), or roots[ranks[i]] for i in [r0, r1)
+ * when ranks is given. */
+static int run_roots(const or_prep *pp, const or_plan *p, const uint64_t *ranks, uint64_t r0, uint64_t r1,
+                     int nthreads, int want_hash, uint32_t *match_buf, uint64_t match_cap, or_result *res)
 {
     memset(res, 0, sizeof(*res));
     int L = p->k - p->lunmat;
     uint32_t maxd = 0;
     for (uint64_t r = r0; r < r1; r++) {
-        uint32_t u = pp->roots[r];
+        uint32_t u = pp->roots[ranks ? ranks[r] : r];
         uint32_t d = (uint32_t)(pp->rowptr[u + 1] - pp->rowptr[u]);
         if (d > maxd) maxd = d;
     }
@@ -760,7 +762,7 @@ static int run_roots(const or_prep *pp, const or_plan *p, uint64_t r0, uint64_t 
         if (!c.E || !c.C) fail = 1;
         else {
 #pragma omp for schedule(dynamic, 1)
-            for (uint64_t r = r0; r < r1; r++) enum_root(&c, pp->roots[r]);
+            for (uint64_t r = r0; r < r1; r++) enum_root(&c, pp->roots[ranks ? ranks[r] : r]);
         }
         free(c.E);
         free(c.C);
@@ -782,7 +784,7 @@ static int run_roots(const or_prep *pp, const or_plan *p, uint64_t r0, uint64_t 
 int or_count(const or_prep *pp, const or_plan *p, int nthreads, int want_hash,
              uint32_t *match_buf, uint64_t match_cap, or_result *res)
 {
-    return run_roots(pp, p, 0, pp->nroots, nthreads, want_hash, match_buf, match_cap, res);
+    return run_roots(pp, p, NULL, 0, pp->nroots, nthreads, want_hash, match_buf, match_cap, res);
 }
 
 int or_count_range(const or_prep *pp, const or_plan *p, uint64_t r0, uint64_t r1, int nthreads,
@@ -790,7 +792,15 @@ int or_count_range(const or_prep *pp, const or_plan *p, uint64_t r0, uint64_t r1
 {
     if (r1 > pp->nroots) r1 = pp->nroots;
     if (r0 > r1) r0 = r1;
-    return run_roots(pp, p, r0, r1, nthreads, 0, NULL, 0, res);
+    return run_roots(pp, p, NULL, r0, r1, nthreads, 0, NULL, 0, res);
+}
+
+int or_count_ranks(const or_prep *pp, const or_plan *p, const uint64_t *ranks, uint64_t n, int nthreads,
+                   int want_hash, or_result *res)
+{
+    for (uint64_t i = 0; i < n; i++)
+        if (ranks[i] >= pp->nroots) return -1;
+    return run_roots(pp, p, ranks, 0, n, nthreads, want_hash, NULL, 0, res);
 }
 
 uint64_t or_alg_bytes(const or_prep *pp)

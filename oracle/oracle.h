@@ -108,6 +108,10 @@ int or_count(const or_prep *pp, const or_plan *p, int nthreads, int want_hash,
 int or_count_range(const or_prep *pp, const or_plan *p, uint64_t r0, uint64_t r1,
                    int nthreads, or_result *res);
 
+/* Count over the roots at the given ranks of the root list. */
+int or_count_ranks(const or_prep *pp, const or_plan *p, const uint64_t *ranks, uint64_t n, int nthreads,
+                   int want_hash, or_result *res);
+
 uint64_t or_tuple_hash(const uint32_t *t, int len);
 
 /* B_alg = sum_{u in R} [4(d(u)+2) + sum_{v in N(u)} 4(d(v)+2)], peeled degrees. */

@@ -74,6 +74,8 @@ def lib():
                                C.c_uint64, C.POINTER(Result)]
         L.or_count_range.argtypes = [C.POINTER(Prep), C.POINTER(Plan), C.c_uint64, C.c_uint64,
                                      C.c_int, C.POINTER(Result)]
+        L.or_count_ranks.argtypes = [C.POINTER(Prep), C.POINTER(Plan), u64p, C.c_uint64, C.c_int, C.c_int,
+                                     C.POINTER(Result)]
         L.or_tuple_hash.argtypes = [u32p, C.c_int]
         L.or_tuple_hash.restype = C.c_uint64
         L.or_alg_bytes.argtypes = [C.POINTER(Prep)]
@@ -194,6 +196,15 @@ class Prepared:
             raise RuntimeError("or_count_range failed")
         return res.count
 
+    def count_ranks(self, ranks, nthreads=0, want_hash=False):
+        """Count over the roots at the given ranks of the root list."""
+        r = np.ascontiguousarray(np.asarray(ranks, dtype=np.uint64))
+        res = Result()
+        if lib().or_count_ranks(C.byref(self.pp), C.byref(self.p), r.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                len(r), nthreads, 1 if want_hash else 0, C.byref(res)):
+            raise RuntimeError("or_count_ranks failed")
+        return dict(count=res.count, hash_sum=res.hash_sum, hash_xor=res.hash_xor)
+
     def alg_bytes(self):
         return lib().or_alg_bytes(C.byref(self.pp))
 
@@ -202,6 +213,35 @@ class Prepared:
             lib().or_prep_free(C.byref(self.pp))
         except Exception:
             pass
+
+
+class PreparedView(Prepared):
+    """An or_prep over arrays prepared elsewhere (rowptr, col, ocol, nlow,
+    roots as numpy arrays, kept alive here): the oracle's enumeration on a
+    graph too large for its own single-threaded preparation.  Only the
+    enumeration is then checked; the preparation is checked on smaller graphs."""
+
+    def __init__(self, p, rowptr, col, ocol, nlow, roots, asc):
+        self.g, self.p = None, p
+        self._keep = [np.ascontiguousarray(rowptr, dtype=np.uint64), np.ascontiguousarray(col, dtype=np.uint32),
+                      np.ascontiguousarray(ocol, dtype=np.uint32), np.ascontiguousarray(nlow, dtype=np.uint32),
+                      np.ascontiguousarray(roots, dtype=np.uint32)]
+        rp, cl, oc, nl, rt = self._keep
+        pp = Prep()
+        pp.n = len(rp) - 1
+        pp.m = int(rp[-1])
+        u32p, u64p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+        pp.rowptr = rp.ctypes.data_as(u64p)
+        pp.col = cl.ctypes.data_as(u32p)
+        pp.ocol = oc.ctypes.data_as(u32p)
+        pp.nlow = nl.ctypes.data_as(u32p)
+        pp.roots = rt.ctypes.data_as(u32p)
+        pp.nroots = len(rt)
+        pp.asc = 1 if asc else 0
+        self.pp = pp
+
+    def __del__(self):
+        pass  # the arrays belong to this object, not to liboracle
 
 
 def tuple_hash(t):

@@ -391,6 +391,57 @@ def test_summed_last_level_degree_boundaries(idx):
     assert r["count"] == o["count"]
 
 
+# Central-node patterns whose last three levels pair up (KPlan::pair): W5
+# (hub + 4-cycle, index 14) and four of 6-7 vertices; tests/test_pair_sum.py
+# lists every qualifying plan and checks the identity the kernels use.
+PAIR_SUMMED = [14, 48, 165, 166, 204]
+
+
+@pytest.mark.parametrize("idx", PAIR_SUMMED)
+@pytest.mark.parametrize("mode", [B.SGM, B.SGC])
+def test_paired_levels_on_hubs(cl_graph, idx, mode):
+    """Count-only runs (paired last levels on LDS batches and hub items)
+    equal the oracle."""
+    g, og = cl_graph
+    say("paired levels", idx, mode)
+    r, o, pp = run_pair(g, og, _central(idx), mode, emit=B.EMIT_NONE)
+    assert pp.arrays()["n_hub_roots"] > 0
+    assert r["count"] == o["count"], (r["count"], o["count"])
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
+def test_paired_levels_lane_widths(cl_graph, lanes):
+    g, og = cl_graph
+    say("paired levels lanes", lanes)
+    for idx in (14, 48):
+        r, o, _ = run_pair(g, og, _central(idx), B.SGM, emit=B.EMIT_NONE, lanes=lanes)
+        assert r["count"] == o["count"], idx
+
+
+@pytest.mark.parametrize("idx", [14, 204])
+def test_paired_levels_degree_boundaries(idx):
+    g = star_plus_graph([63, 64, 65, 127, 128, 129, 255, 256, 511, 512, 513, 700, 1100])
+    og = oracle_graph(g)
+    say("paired levels boundaries", idx)
+    r, o, _ = run_pair(g, og, _central(idx), B.SGM, emit=B.EMIT_NONE)
+    assert r["count"] == o["count"]
+
+
+@pytest.mark.parametrize("donate_min", [2, 16])
+def test_paired_levels_deeper_plans_donate(cl_graph, donate_min):
+    """A 6-vertex paired plan still donates level-2 subtrees; the paired sum
+    then runs on the donated ranges too."""
+    g, og = cl_graph
+    say("paired levels donated", donate_min)
+    s, d = P.directed(_central(48))
+    pp = B.Prepared(g, B.Plan.from_edges(s, d, B.SGM))
+    pp.upload(0)
+    o = O.Prepared(og, O.plan(s, d, 0)).count(want_hash=False)
+    r = pp.count(emit=B.EMIT_NONE, donate=True, donate_min=donate_min)
+    assert r["donations"] > 0
+    assert r["count"] == o["count"]
+
+
 def test_golden_quirk_cases():
     """Root selection where it decides the count (tests/golden/quirks.json:
     a vertex at the root window's upper bound, the ascending head skip with

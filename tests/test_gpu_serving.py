@@ -104,7 +104,7 @@ def test_server_pool_admission(cl_graph):
     say("server pool admission")
     # a pool that holds one hub query's scratch at a time
     probe = B.Server(cl_graph, lanes=1)
-    t = probe.submit(plan_of("w5", B.SGM))
+    t = probe.submit(plan_of("w5", B.SGM), emit=B.EMIT_HASH)  # the footprint of the queries below
     need = probe.poll(timeout_ms=60000)["footprint"]
     probe.close()
     assert need > 0
@@ -120,7 +120,7 @@ def test_server_pool_admission(cl_graph):
         assert r["status"] == 0 and (r["count"], r["hash_sum"]) == (ref["count"], ref["hash_sum"])
     # a query that can never fit is refused, the server keeps serving
     tiny = B.Server(cl_graph, lanes=2, pool_bytes=1 << 20)
-    t_big = tiny.submit(plan_of("w5", B.SGM))
+    t_big = tiny.submit(plan_of("w5", B.SGM), emit=B.EMIT_HASH)
     t_small = tiny.submit(plan_of("triangle", B.SGC))
     out = {}
     for _ in range(2):

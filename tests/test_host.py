@@ -181,3 +181,26 @@ def test_unsorted_rows_are_refused():
         O.Prepared(O.Graph(np.array(src, dtype=np.uint32), np.array(dst, dtype=np.uint32)), O.plan(s, d, 1))
     gs = B.Graph.from_edges([a for a, _ in e], [b for _, b in e])
     assert B.Prepared(gs, plan).arrays()["q0_size"] == 4
+
+
+def test_prep_subset_keeps_ranks_and_counts_add_up():
+    """beep_prep_subset takes the roots at the given ranks of the parent's
+    root list; the oracle's count over ranks adds up to its full count."""
+    g = B.Graph.chung_lu(3000, 9000, 2.2, 3.0, 5)
+    og = oracle_graph(g)
+    for name, mode in (("diamond", B.SGM), ("4clique", B.SGC), ("w5", B.SGM)):
+        s, d = directed(P.read_mtx_edges(name))
+        pp = B.Prepared(g, B.Plan.from_edges(s, d, mode))
+        roots = pp.arrays()["roots"]
+        ranks = np.arange(1, len(roots), 3, dtype=np.uint64)
+        sub = pp.subset(ranks)
+        assert sub.arrays()["roots"].tolist() == roots[ranks.astype(np.int64)].tolist()
+        op = O.Prepared(og, O.plan(s, d, mode))
+        assert op.arrays()["roots"].tolist() == roots.tolist()
+        parts = [np.arange(k, len(roots), 3, dtype=np.uint64) for k in range(3)]
+        total = sum(op.count_ranks(r)["count"] for r in parts)
+        assert total == op.count()["count"]
+    with pytest.raises(B.BeepError):
+        pp.subset([2, 1])
+    with pytest.raises(B.BeepError):
+        pp.subset([len(roots)])
