@@ -90,20 +90,23 @@ struct Scratch {
 };
 void carve(Scratch &s, char *base, const ScratchNeeds &n);
 
-// Streams and events of one launch sequence: hub rounds on `main`, LDS lane
-// classes round-robin on the side streams.
+// Streams and events of one launch sequence: the heaviest LDS lane class on
+// `main` right after the counters are cleared, the hub rounds on side[0], the
+// other lane classes on the side streams (side[1..] when there are more).
 struct Lane {
-    static constexpr int kMaxSide = 3;
+    static constexpr int kMaxSide = 4;
     int device = -1;
     int nside = 0;
     hipStream_t main = nullptr;
     hipStream_t side[kMaxSide] = {};
     hipEvent_t ev[4] = {};  // start, hubs done, all done, counters copied
     hipEvent_t side_done[kMaxSide] = {};
-    // end of each LDS lane class's launch (class c runs on side[c % nside])
+    // end of each LDS lane class's launch, its lane class, and what it started
+    // after (-1: ev[0], -2: ev[1], i >= 0: cls_done[i])
     static constexpr int kMaxClasses = 32;
     hipEvent_t cls_done[kMaxClasses] = {};
     uint8_t cls_lg[kMaxClasses] = {};
+    int8_t cls_after[kMaxClasses] = {};
     int ncls = 0;
     void create(int device, int nside);
     ~Lane();
@@ -137,9 +140,11 @@ struct LaunchCfg {
 };
 
 // Zero fill kernel (no memset nodes, so sequences can be captured) and the
-// kernel that arms a launch sequence's deadlines.
+// first kernel of a launch sequence, which clears n counter words and writes
+// the deadlines at words dl, dl + 1.
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
-hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st);
+hipError_t launch_begin_sequence(unsigned long long *ctr, uint64_t n, uint64_t dl, uint64_t ticks_hub,
+                                 uint64_t ticks_lds, hipStream_t st);
 
 int device_cus(int device);
 uint64_t wall_clock_khz(int device);
@@ -161,8 +166,10 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
 // lane class in cls_lg[i].  Returns the number of classes.
 int enqueue_lds_classes(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStream_t *streams, int nst,
                         hipEvent_t *cls_done = nullptr, uint8_t *cls_lg = nullptr);
-// Device time of each lane class's launch (ms, by lane class) from a lane's events.
+// Device time of each lane class's launch (ms, by lane class) from a lane's
+// events, and of the whole LDS phase (ev[0] to the last class's end).
 void class_times(const Lane &lane, double *t_by_lg);
+double lds_phase_ms(const Lane &lane);
 void fill_result(const beep_opts &o, const unsigned long long *acc, beep_result &res);
 // Balance figures from the busy-time accumulators (kProfPhases x kProfSlots).
 void fill_profile(const unsigned long long *prof, beep_result &res);

@@ -199,7 +199,7 @@ struct LdsBudget {
     static constexpr uint32_t kMaxRoots = 128;     // roots per batch
     static constexpr uint32_t kMaxSumDeg = 512;    // sum of degrees per batch
     static constexpr uint32_t kMaxEWords = 4096;   // bitmap words (u64)
-    static constexpr uint32_t kMaxHash = 1024;     // hash slots
+    static constexpr uint32_t kMaxHash = 2048;     // hash slots
     static constexpr int kMaxClass = 6;
     static constexpr uint32_t ewords(int lg)
     {

@@ -91,8 +91,7 @@ void count_noalloc(Prep &pp, const Plan &p, const beep_opts &o, void *stream, be
 
     unsigned long long acc[kCtrWords];
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrTotal, st));
-    HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
+    HIP_CHECK(launch_begin_sequence(s.ctr, kCtrTotal, kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
     enqueue_lds_classes(pp, c, s, &st, 1);
     enqueue_hubs(pp, c, s, st);
     HIP_CHECK(hipMemcpyAsync(acc, s.ctr, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, st));

@@ -15,10 +15,12 @@
 
 namespace beep {
 
+// Hash slots of a root's row (LDS batches and hub arena): the smallest power
+// of two >= max(4, 4d), as the kernels' hash_slots_dev.
 uint32_t hash_slots_for(uint32_t d)
 {
-    uint32_t h = 2;
-    while (h < 2 * d) h <<= 1;
+    uint32_t h = 4;
+    while (h < 4 * d) h <<= 1;
     return h;
 }
 

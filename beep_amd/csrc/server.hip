@@ -282,11 +282,9 @@ void Server::finish(Slot &sl, int status, const std::string &err)
             float t_all = 0, t_hub = 0;
             HIP_CHECK(hipEventElapsedTime(&t_all, sl.lane.ev[0], sl.lane.ev[3]));
             HIP_CHECK(hipEventElapsedTime(&t_hub, sl.lane.ev[0], sl.lane.ev[1]));
-            float t_lds = 0;
-            HIP_CHECK(hipEventElapsedTime(&t_lds, sl.lane.ev[0], sl.lane.side_done[0]));
             d.result.t_enum_ms = t_all;
             d.result.t_hub_ms = t_hub;
-            d.result.t_lds_ms = t_lds;
+            d.result.t_lds_ms = lds_phase_ms(sl.lane);
             if (sl.c.out.buf && j.opts.match_buf && d.result.n_written) {
                 HIP_CHECK(hipMemcpyAsync(j.opts.match_buf, sl.c.out.buf,
                                          d.result.n_written * (uint64_t)j.plan.L * sizeof(uint32_t),
@@ -360,8 +358,12 @@ void Server::run()
                 running_pilots--;
                 sl.busy_pilot = false;
                 float t_hub = 0, t_lds = 0;
-                const bool ok = hipEventElapsedTime(&t_hub, sl.lane.ev[0], sl.lane.ev[1]) == hipSuccess &&
-                                hipEventElapsedTime(&t_lds, sl.lane.ev[0], sl.lane.side_done[0]) == hipSuccess;
+                bool ok = hipEventElapsedTime(&t_hub, sl.lane.ev[0], sl.lane.ev[1]) == hipSuccess;
+                for (int i = 0; ok && i < sl.lane.ncls; i++) {
+                    float t = 0;
+                    ok = hipEventElapsedTime(&t, sl.lane.ev[0], sl.lane.cls_done[i]) == hipSuccess;
+                    t_lds = std::max(t_lds, t);
+                }
                 // hub and LDS phases run side by side; each scales by its own stride
                 const double t = std::max((double)t_hub * hub_sample_stride(stride) / stride, (double)t_lds);
                 const bool stopped = sl.host_ctr[kCtrStop] != 0;

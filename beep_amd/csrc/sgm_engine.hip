@@ -539,29 +539,112 @@ void class_times(const Lane &lane, double *t_by_lg)
 {
     for (int i = 0; i < 8; i++) t_by_lg[i] = 0;
     for (int i = 0; i < lane.ncls; i++) {
-        // a class starts when the class before it on its stream ends (or at ev[0])
-        const hipEvent_t start = i >= lane.nside ? lane.cls_done[i - lane.nside] : lane.ev[0];
+        const int a = lane.cls_after[i];
+        const hipEvent_t start = a == -1 ? lane.ev[0] : a == -2 ? lane.ev[1] : lane.cls_done[a];
         float t = 0;
         HIP_CHECK(hipEventElapsedTime(&t, start, lane.cls_done[i]));
         if (lane.cls_lg[i] < 8) t_by_lg[lane.cls_lg[i]] += t;
     }
 }
 
+double lds_phase_ms(const Lane &lane)
+{
+    float tmax = 0;
+    for (int i = 0; i < lane.ncls; i++) {
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, lane.ev[0], lane.cls_done[i]));
+        tmax = std::max(tmax, t);
+    }
+    return tmax;
+}
+
+// Lane classes of a query: [batch begin, end) and lane class, with the bitmap
+// build units as the estimate of their work.
+struct LdsClass {
+    uint32_t b0, b1;
+    int lg;
+    uint64_t work;
+};
+
+static std::vector<LdsClass> lds_classes(const Prep &pp, const LaunchCfg &c)
+{
+    std::vector<LdsClass> out;
+    const DevRoots &R = pp.dev->roots;
+    uint32_t b = 0;
+    while (b < R.nbatches) {
+        const int lg = std::max<int>(pp.batch_lanes_log2[b], c.lmin);
+        uint32_t e = b + 1;
+        uint64_t work = 0;
+        while (e < R.nbatches && std::max<int>(pp.batch_lanes_log2[e], c.lmin) == lg) e++;
+        for (int g = 0; g < 8; g++)
+            if (std::max(g, c.lmin) == lg) work += pp.cls_units[g] + 8 * pp.cls_roots[g];
+        out.push_back({b, e, lg, work});
+        b = e;
+    }
+    return out;
+}
+
 void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &lane, unsigned long long *host_ctr)
 {
     hipStream_t st = lane.main;
-    HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrTotal, st));
+    HIP_CHECK(launch_begin_sequence(s.ctr, kCtrTotal, kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
     if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, st));
-    HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
     HIP_CHECK(hipEventRecord(lane.ev[0], st));
-    // The LDS lane classes run on the side streams, concurrently with the hub
-    // rounds on the main stream.
-    for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipStreamWaitEvent(lane.side[i], lane.ev[0], 0));
-    lane.ncls = enqueue_lds_classes(pp, c, s, lane.side, lane.nside, lane.cls_done, lane.cls_lg);
-    for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipEventRecord(lane.side_done[i], lane.side[i]));
-    enqueue_hubs(pp, c, s, st);
-    HIP_CHECK(hipEventRecord(lane.ev[1], st));
-    for (int i = 0; i < lane.nside; i++) HIP_CHECK(hipStreamWaitEvent(st, lane.side_done[i], 0));
+    // The heaviest LDS lane class starts on the main stream at once (a wait on
+    // another stream's event costs tens of microseconds); the hub rounds run on
+    // side[0] beside it; the other classes go, heaviest first, to the stream
+    // with the least work queued (side[1..], or main and side[0] when there
+    // is only one side stream).
+    std::vector<LdsClass> cls = c.kp.L >= 3 ? lds_classes(pp, c) : std::vector<LdsClass>();
+    if (cls.size() > (size_t)Lane::kMaxClasses) throw Error(BEEP_ERANGE, "too many lane classes");
+    std::vector<int> order(cls.size());
+    for (size_t i = 0; i < cls.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a].work > cls[b].work; });
+    const DevRoots &R = pp.dev->roots;
+    const bool hubs = c.kp.L >= 3 && !R.rounds.empty();
+    // stream k: 0 = main, 1 + i = side[i]
+    const int nstreams = 1 + lane.nside;
+    std::vector<uint64_t> queued(nstreams, 0);
+    std::vector<int> last(nstreams, -1);  // what the next launch on stream k starts after
+    std::vector<bool> waited(nstreams, false);
+    waited[0] = true;
+    auto side_stream = [&](int k) -> hipStream_t {
+        if (!waited[k]) {
+            HIP_CHECK(hipStreamWaitEvent(lane.side[k - 1], lane.ev[0], 0));
+            waited[k] = true;
+        }
+        return lane.side[k - 1];
+    };
+    lane.ncls = 0;
+    for (size_t n = 0; n < order.size(); n++) {
+        const LdsClass &lc = cls[order[n]];
+        int k = 0;
+        if (n > 0) {
+            const int first_lds_side = nstreams > 2 ? 2 : 0;  // keep side[0] for the hubs when possible
+            k = first_lds_side;
+            for (int q = first_lds_side; q < nstreams; q++)
+                if (queued[q] < queued[k]) k = q;
+            if (nstreams <= 2 && hubs && k == 1) k = 0;  // one side stream: it carries the hubs
+        }
+        hipStream_t sk = k == 0 ? st : side_stream(k);
+        const int i = lane.ncls++;
+        unsigned long long *cur = s.ctr + kCtrLds + (uint64_t)order[n] * kCurWords;
+        HIP_CHECK(launch_lds_batches(lc.lg, pp.dev->cus, sk, c.kg, c.kp, R.recs, R.batch_first, R.batch_count,
+                                     lc.b0, lc.b1, cur, c.out, nullptr));
+        HIP_CHECK(hipEventRecord(lane.cls_done[i], sk));
+        lane.cls_lg[i] = (uint8_t)lc.lg;
+        lane.cls_after[i] = (int8_t)last[k];
+        last[k] = i;
+        queued[k] += lc.work;
+    }
+    hipStream_t hs = side_stream(1);
+    if (hubs) enqueue_hubs(pp, c, s, hs);
+    HIP_CHECK(hipEventRecord(lane.ev[1], hs));
+    for (int k = 1; k < nstreams; k++)
+        if (waited[k]) {
+            HIP_CHECK(hipEventRecord(lane.side_done[k - 1], lane.side[k - 1]));
+            HIP_CHECK(hipStreamWaitEvent(st, lane.side_done[k - 1], 0));
+        }
     HIP_CHECK(hipEventRecord(lane.ev[2], st));
     HIP_CHECK(hipMemcpyAsync(host_ctr, s.ctr, sizeof(unsigned long long) * kCtrWords, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipEventRecord(lane.ev[3], st));
@@ -649,18 +732,13 @@ void count_on_device(Prep &pp, const Plan &p, const beep_opts &o, beep_result &r
     unsigned long long acc[kCtrWords];
     enqueue_query(pp, c, d.scratch, d.lane, acc);
     HIP_CHECK(hipStreamSynchronize(d.lane.main));
-    float t_all = 0, t_hub = 0, t_lds = 0;
+    float t_all = 0, t_hub = 0;
     HIP_CHECK(hipEventElapsedTime(&t_all, d.lane.ev[0], d.lane.ev[3]));
     HIP_CHECK(hipEventElapsedTime(&t_hub, d.lane.ev[0], d.lane.ev[1]));
-    for (int i = 0; i < d.lane.nside; i++) {
-        float t = 0;
-        HIP_CHECK(hipEventElapsedTime(&t, d.lane.ev[0], d.lane.side_done[i]));
-        t_lds = std::max(t_lds, t);
-    }
     fill_result(o, acc, res);
     res.t_enum_ms = t_all;
     res.t_hub_ms = t_hub;
-    res.t_lds_ms = t_lds;
+    res.t_lds_ms = lds_phase_ms(d.lane);
     class_times(d.lane, res.t_lds_class_ms);
     if (c.out.buf && o.match_buf && res.n_written)
         HIP_CHECK(hipMemcpy(o.match_buf, c.out.buf, res.n_written * (uint64_t)p.L * sizeof(uint32_t),
@@ -790,10 +868,9 @@ static void batch_capture(Batch &b)
     HIP_CHECK(hipStreamBeginCapture(b.s0, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < n; i++) {
         const Scratch &s = b.scr[i];
-        HIP_CHECK(launch_fill_zero(s.ctr, sizeof(unsigned long long) * kCtrTotal, b.s0));
+        HIP_CHECK(launch_begin_sequence(s.ctr, kCtrTotal, kCtrDeadline, b.cfgs[i].deadline_ticks,
+                                        b.cfgs[i].deadline_ticks_lds, b.s0));
         if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, b.s0));
-        HIP_CHECK(launch_arm_deadline(s.ctr + kCtrDeadline, b.cfgs[i].deadline_ticks, b.cfgs[i].deadline_ticks_lds,
-                                      b.s0));
     }
     HIP_CHECK(launch_fill_zero(b.cursor, kCurWords * 8, b.s0));
     // The persistent LDS kernel runs first, on the whole GPU; the hub branches

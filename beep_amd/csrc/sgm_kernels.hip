@@ -98,11 +98,12 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t v, uint32_t H)
     return h & (H - 1u);
 }
 
-// smallest power of two >= max(2, 2d)
+// smallest power of two >= max(4, 4d): at most a quarter of the slots are
+// taken, so a probe for an absent key rarely goes past its first slot
 __host__ __device__ __forceinline__ uint32_t hash_slots_dev(uint32_t d)
 {
-    if (d <= 1) return 2;
-    return 1u << (32 - __builtin_clz(2 * d - 1));
+    if (d <= 1) return 4;
+    return 1u << (32 - __builtin_clz(4 * d - 1));
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -579,20 +580,20 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
         const bool own = w < W;
         sX[1 * kBlock + tid] = (uint16_t)jl;
 
-        auto level_word = [&](int l) -> uint64_t {
+        auto level_word = [&](int l) -> uint64_t {  // masks first; no row reads once empty
             uint64_t t = valid_mask(w, d);
-            uint32_t qm = sMisc[M_QM + l];
-            while (qm) {
-                const int q = __builtin_ctz(qm);
-                qm &= qm - 1;
-                t &= E[xget(q) * W + w];
-            }
             for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
             uint32_t sm = sMisc[M_SM + l];
             while (sm) {
                 const int s = __builtin_ctz(sm);
                 sm &= sm - 1;
                 t &= gt_mask(xget(s), w);
+            }
+            uint32_t qm = sMisc[M_QM + l];
+            while (qm && t) {
+                const int q = __builtin_ctz(qm);
+                qm &= qm - 1;
+                t &= E[xget(q) * W + w];
             }
             return t;
         };
@@ -635,12 +636,25 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
 
         // Paired last levels (pl.pair), l = L-2 = b: S = C_{l-1} = C_a with
         // x_1..x_{a-1} fixed; adds C(|S n E[x_b]|, 2) over x_b in B, the
-        // candidates of level b without its constraints on x_a.
+        // candidates of level b without its constraints on x_a.  Only an x_b
+        // adjacent to some member of S can add anything, so B is first cut to
+        // the union of their rows.
         auto pair_sum = [&](int l) -> uint64_t {
             const uint64_t S = own ? sStk[(l - 3) * kBlock + tid] : 0ull;
+            uint64_t U = 0;
+            for (uint64_t rem = S;;) {
+                const uint64_t bal = __ballot(rem != 0ull);
+                const uint64_t mine = (P == 64) ? bal : ((bal >> pbase) & ((1ull << P) - 1ull));
+                if (mine == 0ull) break;
+                const int f = __builtin_ctzll(mine);
+                const uint64_t vf = __shfl(rem, f, P);
+                const uint32_t xs = ((uint32_t)f << 6) + (uint32_t)__builtin_ctzll(vf);
+                if (lx == f) rem &= rem - 1ull;
+                if (own) U |= E[xs * W + w];
+            }
             uint64_t B = 0;
             if (own) {
-                B = valid_mask(w, d);
+                B = valid_mask(w, d) & U;
                 uint32_t qm = sMisc[M_QM + l] & ~(1u << (l - 1));
                 while (qm) {
                     const int q = __builtin_ctz(qm);
@@ -1053,21 +1067,26 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         }
         return lo;
     };
+    // The masks go first, then the rows (x_1's from LDS first), and a lane
+    // whose word is already empty loads no further row word.
     auto level_word = [&](int l, uint32_t w) -> uint64_t {
         uint64_t t = valid_mask(w, d);
-        uint32_t qm = sQM[l];
-        while (qm) {
-            const int q = __builtin_ctz(qm);
-            qm &= qm - 1;
-            if (q == 1 && row1) t &= e1[w];
-            else t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
-        }
         for (int s = 1; s < l; s++) t &= ~bit_at(xget(s), w);
         uint32_t sm = sSM[l];
         while (sm) {
             const int s = __builtin_ctz(sm);
             sm &= sm - 1;
             t &= gt_mask(xget(s), w);
+        }
+        uint32_t qm = sQM[l];
+        if (row1 && (qm & 2u)) {
+            t &= e1[w];
+            qm &= ~2u;
+        }
+        while (qm && t) {
+            const int q = __builtin_ctz(qm);
+            qm &= qm - 1;
+            t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
         }
         return t;
     };
@@ -1137,12 +1156,42 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         uint64_t *Bv = my + (uint64_t)(l - 2) * wmax;
         const uint32_t blo = word_lo(l);
         const uint32_t slo = sWlo[wv][l - 1];
+        // only an x_b adjacent to some member of S adds anything: B starts as
+        // the union of their rows (from word blo on)
+        for (uint32_t rr = blo >> 6; rr < R; rr++) {
+            const uint32_t w = rr * 64 + lane;
+            if (w < W && w >= blo) Bv[w] = 0ull;
+        }
+        for (uint32_t pos = slo << 6;;) {
+            uint32_t xs = 0xffffffffu;
+            const uint32_t pw = pos >> 6;
+            for (uint32_t rr = pw >> 6; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                uint64_t v = (w < W && w >= pw) ? S[w] : 0ull;
+                if (w == pw) v &= (~0ull << (pos & 63u));
+                const uint64_t bal = __ballot(v != 0ull);
+                if (bal) {
+                    const int f = __builtin_ctzll(bal);
+                    const uint64_t vf = __shfl(v, f, 64);
+                    xs = ((rr * 64 + (uint32_t)f) << 6) + (uint32_t)__builtin_ctzll(vf);
+                    break;
+                }
+            }
+            if (xs == 0xffffffffu) break;
+            const uint64_t *Es = Eh + (uint64_t)(xs - rlo) * W;  // x_a's row: kept, as the walk reads it
+            for (uint32_t rr = blo >> 6; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                if (w < W && w >= blo) Bv[w] |= Es[w];
+            }
+            pos = xs + 1u;
+        }
+        __builtin_amdgcn_wave_barrier();
         for (uint32_t rr = 0; rr < R; rr++) {
             const uint32_t w = rr * 64 + lane;
             if (w >= W) break;
             uint64_t t = 0;
             if (w >= blo) {
-                t = valid_mask(w, d);
+                t = valid_mask(w, d) & Bv[w];
                 uint32_t qm = sQM[l] & ~(1u << (l - 1));
                 while (qm) {
                     const int q = __builtin_ctz(qm);
@@ -1182,7 +1231,10 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             uint32_t pc = 0;
             for (uint32_t rr = slo >> 6; rr < R; rr++) {
                 const uint32_t w = rr * 64 + lane;
-                if (w < W && w >= slo) pc += (uint32_t)__popcll(S[w] & Eb[w]);
+                if (w < W && w >= slo) {
+                    const uint64_t sw = S[w];
+                    if (sw) pc += (uint32_t)__popcll(sw & Eb[w]);  // x_b's row only where S has members
+                }
             }
             pc = (uint32_t)wave_sum_u64(pc);
             if (lane == 0) cnt += (uint64_t)pc * (pc - 1u) / 2u;
@@ -1478,20 +1530,28 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st)
     return hipGetLastError();
 }
 
-// Writes the deadlines of one launch sequence (hub kernels, LDS kernels):
-// now + ticks, or none.
-__global__ void arm_deadline_kernel(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds)
+// First kernel of a launch sequence: clears the n counter words and writes
+// the deadlines of the hub and the LDS kernels at words dl and dl + 1:
+// now + ticks, or none (all ones).
+__global__ void begin_sequence_kernel(unsigned long long *ctr, uint64_t n, uint64_t dl, uint64_t ticks_hub,
+                                      uint64_t ticks_lds)
 {
-    if (threadIdx.x == 0) {
-        const unsigned long long now = (unsigned long long)wall_clock64();
-        dl[0] = ticks_hub ? now + ticks_hub : ~0ull;
-        dl[1] = ticks_lds ? now + ticks_lds : ~0ull;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        unsigned long long v = 0;
+        if (i == dl || i == dl + 1) {
+            const uint64_t t = i == dl ? ticks_hub : ticks_lds;
+            v = t ? (unsigned long long)wall_clock64() + t : ~0ull;
+        }
+        ctr[i] = v;
     }
 }
 
-hipError_t launch_arm_deadline(unsigned long long *dl, uint64_t ticks_hub, uint64_t ticks_lds, hipStream_t st)
+hipError_t launch_begin_sequence(unsigned long long *ctr, uint64_t n, uint64_t dl, uint64_t ticks_hub,
+                                 uint64_t ticks_lds, hipStream_t st)
 {
-    hipLaunchKernelGGL(arm_deadline_kernel, dim3(1), dim3(64), 0, st, dl, ticks_hub, ticks_lds);
+    const uint64_t blocks = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(begin_sequence_kernel, dim3((uint32_t)(blocks ? blocks : 1)), dim3(kBlock), 0, st, ctr, n, dl,
+                       ticks_hub, ticks_lds);
     return hipGetLastError();
 }
 
