@@ -113,6 +113,7 @@ class Plan:
         self.qedge = [list(d.qedge[d.qedge_ptr[i]:d.qedge_ptr[i + 1]]) for i in range(k)]
         self.sym = [list(d.sym[d.sym_ptr[i]:d.sym_ptr[i + 1]]) for i in range(k)]
         self.width = k - self.lunmat
+        self.upper_rows = bool(d.upper_rows)
 
     @classmethod
     def from_mtx(cls, path, mode):

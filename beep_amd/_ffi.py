@@ -22,7 +22,7 @@ class PlanDesc(C.Structure):
                 ("max_qdeg", C.c_int), ("min_qdeg", C.c_int),
                 ("seq", C.c_uint32 * MAXK), ("qdeg", C.c_uint32 * MAXK),
                 ("qedge_ptr", C.c_uint32 * (MAXK + 1)), ("qedge", C.c_uint32 * 45),
-                ("sym_ptr", C.c_uint32 * (MAXK + 1)), ("sym", C.c_uint32 * 45)]
+                ("sym_ptr", C.c_uint32 * (MAXK + 1)), ("sym", C.c_uint32 * 45), ("upper_rows", C.c_int)]
 
 
 class PrepDesc(C.Structure):

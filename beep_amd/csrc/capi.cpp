@@ -239,6 +239,7 @@ int beep_plan_describe(const beep_plan *p, beep_plan_desc *o)
         memcpy(o->qedge, P.qedge, sizeof(P.qedge));
         memcpy(o->sym_ptr, P.sym_ptr, sizeof(P.sym_ptr));
         memcpy(o->sym, P.sym, sizeof(P.sym));
+        o->upper_rows = P.upper_rows;
     });
 }
 

@@ -94,7 +94,7 @@ void carve(Scratch &s, char *base, const ScratchNeeds &n);
 // `main` right after the counters are cleared, the hub rounds on side[0], the
 // other lane classes on the side streams (side[1..] when there are more).
 struct Lane {
-    static constexpr int kMaxSide = 4;
+    static constexpr int kMaxSide = 3;  // with main: the 4 hardware queues a process gets by default
     int device = -1;
     int nside = 0;
     hipStream_t main = nullptr;

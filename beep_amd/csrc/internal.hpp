@@ -104,6 +104,9 @@ struct Plan {
     int rows_above_split = 0;
     // Some level intersects with a bitmap row (QEDGE(l)[1:] non-empty).
     int uses_bitmap = 0;
+    // Every row x_q a level intersects with is used above x_q only: of each
+    // adjacent pair, bitmaps need the bit in the row of the smaller index.
+    int upper_rows = 0;
     uint16_t qmask[kMaxK] = {};  // positions QEDGE(l)[1:]
     uint16_t smask[kMaxK] = {};  // positions s in SYM(l), s > 0
 };
@@ -163,6 +166,7 @@ struct Prep {
     int skip_split = 0;               // plan's x_1 < split skip
     int rows_above_split = 0;         // hub bitmaps keep rows >= split only
     int uses_bitmap = 1;              // hub bitmaps are built at all
+    int upper_rows = 0;               // bitmaps hold no mirrored bits
     std::vector<uint32_t> wu, woff, wlen;
     uint32_t n_hub = 0;
     std::vector<LdsBatch> batches;

@@ -202,6 +202,30 @@ std::unique_ptr<Plan> plan_from_edges(const uint32_t *src, const uint32_t *dst, 
     p->uses_bitmap = 0;
     for (int l = 2; l < p->L; l++)
         if (p->qmask[l]) p->uses_bitmap = 1;
+    // Whenever level l intersects with the row of x_q (q in QEDGE(l)[1:]),
+    // its candidates are also bound above some x_s, s in SYM(l), with
+    // x_s >= x_q (s = q, or x_s > x_q through the chain of symmetry
+    // constraints): only the bits of row x_q above x_q are ever used, so of
+    // each adjacent pair the bitmaps need the bit in the row of the smaller
+    // index only.
+    bool gt[kMaxK][kMaxK] = {};  // gt[a][b]: x_a > x_b for every candidate tuple
+    for (int l = 1; l < p->L; l++)
+        for (int s = 1; s < l; s++)
+            if ((p->smask[l] >> s) & 1u) {
+                gt[l][s] = true;
+                for (int b = 1; b < s; b++)
+                    if (gt[s][b]) gt[l][b] = true;
+            }
+    bool upper = p->uses_bitmap != 0;
+    for (int l = 2; l < p->L; l++)
+        for (int q = 1; q < l; q++) {
+            if (!((p->qmask[l] >> q) & 1u)) continue;
+            bool ok = false;
+            for (int s = 1; s < l; s++)
+                if (((p->smask[l] >> s) & 1u) && (s == q || gt[s][q])) ok = true;
+            if (!ok) upper = false;
+        }
+    p->upper_rows = upper ? 1 : 0;
     return p;
 }
 

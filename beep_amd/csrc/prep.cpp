@@ -270,6 +270,7 @@ void build_work(Prep &pp_, const Plan &p, uint32_t sample_stride)
     pp->skip_split = p.skip_split;
     pp->rows_above_split = p.rows_above_split;
     pp->uses_bitmap = p.uses_bitmap;
+    pp->upper_rows = p.upper_rows;
     {
         auto len_of = [&](uint32_t r) {
             uint32_t u = pp->roots[r];

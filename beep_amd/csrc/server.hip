@@ -128,7 +128,7 @@ struct Server {
         Scratch s;
         LaunchCfg c;
     };
-    using RootKey = std::tuple<int, int, int, int, int, int, int>;
+    using RootKey = std::tuple<int, int, int, int, int, int, int, int>;
 
     const Graph *g = nullptr;
     int device = 0, cus = 0;
@@ -178,7 +178,7 @@ std::shared_ptr<Prep> Server::get_prep(const Plan &p, uint32_t sample_stride)
 {
     std::lock_guard<std::mutex> lk(cache_mu);
     const RootKey key{p.min_qdeg, p.asc, p.max_qdeg, p.above_split, p.rows_above_split, p.uses_bitmap,
-                      p.skip_split};
+                      p.upper_rows, p.skip_split};
     auto &cache = sample_stride > 1 ? pilotq : rootq;
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;

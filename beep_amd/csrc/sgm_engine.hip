@@ -30,7 +30,7 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                              const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
-                             const uint64_t *kv, uint64_t *E, const KOut &out);
+                             const uint64_t *kv, uint64_t *E, int upper, const KOut &out);
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
@@ -162,8 +162,9 @@ static void upload_variant(Variant &V, int device)
 // larger-key range.  A row below the first kept row (rlo) sets bits only in
 // kept columns, i.e. for entries not before A[rlo] in the row order (every
 // row is sorted by the same key), so its range is cut to that suffix; with
-// no kept row at all it is empty.
-static std::vector<uint64_t> hub_scan_ranges(const Variant &V, const std::vector<HubRec> &hubs)
+// no kept row at all it is empty.  Plans that keep only the bit in the row of
+// the smaller index (upper) need nothing from such a row: it is not scanned.
+static std::vector<uint64_t> hub_scan_ranges(const Variant &V, const std::vector<HubRec> &hubs, bool upper)
 {
     uint64_t rows = 0;
     for (const auto &h : hubs) rows += h.d;
@@ -182,7 +183,7 @@ static std::vector<uint64_t> hub_scan_ranges(const Variant &V, const std::vector
                 uint64_t beg = V.rowptr[v] + (V.asc ? V.nlow[v] : 0u);
                 uint64_t len = V.hlen[v];
                 if (j < h.rlo) {
-                    if (h.rlo >= h.d) {
+                    if (h.rlo >= h.d || upper) {
                         len = 0;
                     } else {
                         const uint32_t p = A[h.rlo];
@@ -313,7 +314,7 @@ void upload(Prep &pp, int device, int p_skip_split, uint64_t arena_budget)
             R.arena_words = std::max(R.arena_words, r.ewords);
             R.arena_slots = std::max(R.arena_slots, r.hslots);
         }
-        if (pp.uses_bitmap) R.hub_range = dev_upload(hub_scan_ranges(V, hubs));
+        if (pp.uses_bitmap) R.hub_range = dev_upload(hub_scan_ranges(V, hubs, pp.upper_rows != 0));
         R.hubs = dev_upload(hubs);
         R.hub_of_row = dev_upload(hor);
         R.hub_of_item = dev_upload(hoi);
@@ -419,6 +420,7 @@ static KPlan make_kplan(const Plan &p, int emit, int build_mode)
     k.bin_ratio = build_mode == BEEP_BUILD_SCAN ? ~0u : build_mode == BEEP_BUILD_SEARCH ? 0u : 32u;
     k.tail = tail_summed(p, emit) ? 1 : 0;
     k.pair = pair_summed(p, emit) ? 1 : 0;
+    k.upper = p.upper_rows;
     for (int l = 0; l < kMaxLevels; l++) {
         k.qmask[l] = p.qmask[l];
         k.smask[l] = p.smask[l];
@@ -432,7 +434,8 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     const Variant &V = *pp.var;
     if (p.min_qdeg != V.min_qdeg || p.max_qdeg != pp.max_qdeg || p.asc != V.asc ||
         p.above_split != pp.reduced || p.rows_above_split != pp.rows_above_split ||
-        p.uses_bitmap != pp.uses_bitmap || p.skip_split != pp.dev->roots.skip_split)
+        p.uses_bitmap != pp.uses_bitmap || p.upper_rows != pp.upper_rows ||
+        p.skip_split != pp.dev->roots.skip_split)
         throw Error(BEEP_EINVAL, "plan does not match the prepared graph");
     const DevRoots &R = pp.dev->roots;
     LaunchCfg c{};
@@ -489,7 +492,7 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
             HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv));
             int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
             HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, R.hub_range, r.g0, r.g1, s.kv,
-                                        s.arena, c.out));
+                                        s.arena, c.kp.upper, c.out));
         }
         unsigned long long *cur = s.ctr + kCtrHub + (uint64_t)(ri % 64) * kCurWords;
         if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, kCurWords * 8, st));
@@ -591,20 +594,24 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
     if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, st));
     HIP_CHECK(hipEventRecord(lane.ev[0], st));
     // The heaviest LDS lane class starts on the main stream at once (a wait on
-    // another stream's event costs tens of microseconds); the hub rounds run on
-    // side[0] beside it; the other classes go, heaviest first, to the stream
-    // with the least work queued (side[1..], or main and side[0] when there
-    // is only one side stream).
+    // another stream's event costs tens of microseconds); the hub rounds, the
+    // longest chain of dependent kernels, are queued next, on side[0]; the
+    // other classes follow widest first (their batches take longest, so they
+    // start early and the narrow batches fill in around them), one on each
+    // further side stream.  Classes left over queue on main behind the first
+    // one, which holds the CUs' LDS while it runs: they start as it drains.
+    // Without hubs, side[0] takes a class too.
     std::vector<LdsClass> cls = c.kp.L >= 3 ? lds_classes(pp, c) : std::vector<LdsClass>();
     if (cls.size() > (size_t)Lane::kMaxClasses) throw Error(BEEP_ERANGE, "too many lane classes");
     std::vector<int> order(cls.size());
     for (size_t i = 0; i < cls.size(); i++) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a].work > cls[b].work; });
+    if (order.size() > 2)
+        std::stable_sort(order.begin() + 1, order.end(), [&](int a, int b) { return cls[a].lg > cls[b].lg; });
     const DevRoots &R = pp.dev->roots;
     const bool hubs = c.kp.L >= 3 && !R.rounds.empty();
     // stream k: 0 = main, 1 + i = side[i]
     const int nstreams = 1 + lane.nside;
-    std::vector<uint64_t> queued(nstreams, 0);
     std::vector<int> last(nstreams, -1);  // what the next launch on stream k starts after
     std::vector<bool> waited(nstreams, false);
     waited[0] = true;
@@ -616,16 +623,12 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
         return lane.side[k - 1];
     };
     lane.ncls = 0;
+    hipStream_t hs = nullptr;
     for (size_t n = 0; n < order.size(); n++) {
         const LdsClass &lc = cls[order[n]];
-        int k = 0;
-        if (n > 0) {
-            const int first_lds_side = nstreams > 2 ? 2 : 0;  // keep side[0] for the hubs when possible
-            k = first_lds_side;
-            for (int q = first_lds_side; q < nstreams; q++)
-                if (queued[q] < queued[k]) k = q;
-            if (nstreams <= 2 && hubs && k == 1) k = 0;  // one side stream: it carries the hubs
-        }
+        // class n >= 1 on side stream first_side + n - 1 while there is one
+        const int first_side = hubs ? 2 : 1;
+        const int k = n > 0 && first_side + (int)n - 1 < nstreams ? first_side + (int)n - 1 : 0;
         hipStream_t sk = k == 0 ? st : side_stream(k);
         const int i = lane.ncls++;
         unsigned long long *cur = s.ctr + kCtrLds + (uint64_t)order[n] * kCurWords;
@@ -635,11 +638,17 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
         lane.cls_lg[i] = (uint8_t)lc.lg;
         lane.cls_after[i] = (int8_t)last[k];
         last[k] = i;
-        queued[k] += lc.work;
+        if (n == 0) {
+            hs = side_stream(1);
+            if (hubs) enqueue_hubs(pp, c, s, hs);
+            HIP_CHECK(hipEventRecord(lane.ev[1], hs));
+        }
     }
-    hipStream_t hs = side_stream(1);
-    if (hubs) enqueue_hubs(pp, c, s, hs);
-    HIP_CHECK(hipEventRecord(lane.ev[1], hs));
+    if (!hs) {
+        hs = side_stream(1);
+        if (hubs) enqueue_hubs(pp, c, s, hs);
+        HIP_CHECK(hipEventRecord(lane.ev[1], hs));
+    }
     for (int k = 1; k < nstreams; k++)
         if (waited[k]) {
             HIP_CHECK(hipEventRecord(lane.side_done[k - 1], lane.side[k - 1]));

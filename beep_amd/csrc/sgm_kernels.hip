@@ -26,6 +26,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "internal.hpp"
 #include "sgm_kernels.hpp"
 
@@ -96,6 +98,15 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t v, uint32_t H)
     uint32_t h = v * 0x9E3779B1u;
     h ^= h >> 15;
     return h & (H - 1u);
+}
+
+// An adjacent pair (j, i) of an LDS neighbourhood bitmap (W words per row):
+// both bits, or with `upper` only the one in the row of the smaller index
+// (the symmetry bounds never read a row below its own index then).
+__device__ __forceinline__ void set_pair(uint64_t *E, uint32_t W, uint32_t j, uint32_t i, int upper)
+{
+    if (!upper || i > j) atomicOr((unsigned long long *)&E[j * W + (i >> 6)], 1ull << (i & 63));
+    if (!upper || j > i) atomicOr((unsigned long long *)&E[i * W + (j >> 6)], 1ull << (j & 63));
 }
 
 // smallest power of two >= max(4, 4d): at most a quarter of the slots are
@@ -530,10 +541,7 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                         const uint32_t x = g.ocol[hb + m];
                         if (row_before(g.asc, g.vinfo[x].deg, x, da, a)) blo = m + 1; else bhi = m;
                     }
-                    if (blo < hl && g.ocol[hb + blo] == a) {
-                        atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
-                        atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
-                    }
+                    if (blo < hl && g.ocol[hb + blo] == a) set_pair(sE + rr.eoff, W, jl, i, pl.upper);
                     continue;
                 }
                 const uint32_t w = ent[k];
@@ -541,10 +549,9 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                 for (;;) {
                     const uint32_t key = sHKey[rr.hoff + s];
                     if (key == 0u) break;
-                    if (key == w + 1u) {
-                        const uint32_t i = sHVal[rr.hoff + s];
-                        atomicOr((unsigned long long *)&sE[rr.eoff + jl * W + (i >> 6)], 1ull << (i & 63));
-                        atomicOr((unsigned long long *)&sE[rr.eoff + i * W + (jl >> 6)], 1ull << (jl & 63));
+                    if (key == w + 1u) {  // a row holds each vertex once
+                        set_pair(sE + rr.eoff, W, jl, sHVal[rr.hoff + s], pl.upper);
+                        break;
                     }
                     s = (s + 1) & (H - 1);
                 }
@@ -862,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                                                             const uint64_t *__restrict__ hub_range,
                                                             uint64_t g0, uint64_t g1,
                                                             const uint64_t *__restrict__ kv,
-                                                            uint64_t *E, KOut out)
+                                                            uint64_t *E, int upper, KOut out)
 {
     __shared__ uint64_t sBeg[kBlock];
     __shared__ uint32_t sPre[kBlock];
@@ -959,12 +966,15 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
                     if (key == 0u) break;
                     if (key == wv2 + 1u) {
                         const uint32_t i = (uint32_t)(slot >> 32);
-                        if (jl >= h.rlo)
+                        // upper plans keep the bit in the row of the smaller index only
+                        const bool fwd = !upper || i > jl, mir = !upper || jl > i;
+                        if (fwd && jl >= h.rlo)
                             atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(jl - h.rlo) * W + (i >> 6)],
                                      1ull << (i & 63));
-                        if (i >= h.rlo)
+                        if (mir && i >= h.rlo)
                             atomicOr((unsigned long long *)&E[h.eoff + (uint64_t)(i - h.rlo) * W + (jl >> 6)],
                                      1ull << (jl & 63));
+                        break;  // a hub row holds each vertex once
                     }
                     s = (s + 1) & (h.hslots - 1);
                     slot = kv[h.hoff + s];
@@ -1464,20 +1474,40 @@ static hipError_t launch_lds_p(int grid, size_t lds, hipStream_t st, const KGrap
     return hipGetLastError();
 }
 
+// Workgroups of a kernel per CU with `lds` bytes of dynamic LDS, and its
+// dynamic-LDS attribute raised to at least `lds`.  Both are host calls of
+// several microseconds on the launch path of every query, so they are made
+// once per (kernel, LDS size); the attribute only ever grows.
+struct LdsLaunchCache {
+    std::mutex mu;
+    size_t attr = 0;
+    size_t lds[16] = {};
+    int occ[16] = {};
+    int n = 0;
+    int get(const void *fn, size_t bytes)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (int i = 0; i < n; i++)
+            if (lds[i] == bytes) return occ[i];
+        if (bytes > attr) {
+            (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            attr = bytes;
+        }
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kBlock, bytes) != hipSuccess || nb < 1) nb = 1;
+        if (n < 16) {
+            lds[n] = bytes;
+            occ[n++] = nb;
+        }
+        return nb;
+    }
+};
+
 template <int P>
 static int occupancy_lds_p(size_t lds)
 {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lds_batch_kernel<P>, kBlock, lds) != hipSuccess)
-        return 1;
-    return nb > 0 ? nb : 1;
-}
-
-template <int P>
-static void set_lds_attr(size_t lds)
-{
-    (void)hipFuncSetAttribute((const void *)lds_batch_kernel<P>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static LdsLaunchCache cache;
+    return cache.get((const void *)lds_batch_kernel<P>, lds);
 }
 
 // Dispatch on the DFS group width.
@@ -1493,7 +1523,6 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
     switch (lg) {
 #define BEEP_CASE(LG)                                   \
     case LG:                                            \
-        set_lds_attr<(1 << LG)>(lds);                   \
         occ = occupancy_lds_p<(1 << LG)>(lds);          \
         break;
         BEEP_CASE(0) BEEP_CASE(1) BEEP_CASE(2) BEEP_CASE(3) BEEP_CASE(4) BEEP_CASE(5) BEEP_CASE(6)
@@ -1581,10 +1610,10 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
 
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                              const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
-                             const uint64_t *kv, uint64_t *E, const KOut &out)
+                             const uint64_t *kv, uint64_t *E, int upper, const KOut &out)
 {
     hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, hrange, g0, g1, kv, E,
-                       out);
+                       upper, out);
     return hipGetLastError();
 }
 
@@ -1594,6 +1623,12 @@ constexpr size_t kHubLdsMax = 96 * 1024;
 
 bool hub_dfs_in_lds(int nlev, uint32_t wmax) { return hub_lds_bytes(nlev, wmax) <= kHubLdsMax; }
 
+static LdsLaunchCache &hub_dfs_lds_cache()
+{
+    static LdsLaunchCache c;
+    return c;
+}
+
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
@@ -1602,8 +1637,7 @@ hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan
     const int nlev = pl.L > 3 ? pl.L - 3 : 1;
     if (hub_dfs_in_lds(nlev, wmax)) {
         const size_t lds = hub_lds_bytes(nlev, wmax);
-        (void)hipFuncSetAttribute((const void *)hub_dfs_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
+        (void)hub_dfs_lds_cache().get((const void *)hub_dfs_kernel<true>, lds);
         hipLaunchKernelGGL(hub_dfs_kernel<true>, dim3(grid), dim3(kBlock), lds, st, g, pl, hubs, hoi, i0, i1,
                            cursor, E, stk, wmax, out, hq, mode);
     } else {
@@ -1615,15 +1649,10 @@ hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan
 
 int hub_dfs_occupancy(int nlev, uint32_t wmax)
 {
-    int nb = 0;
-    hipError_t e;
     if (hub_dfs_in_lds(nlev, wmax))
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hub_dfs_kernel<true>, kBlock,
-                                                         hub_lds_bytes(nlev, wmax));
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hub_dfs_kernel<false>, kBlock, 0);
-    if (e != hipSuccess) return 1;
-    return nb > 0 ? nb : 1;
+        return hub_dfs_lds_cache().get((const void *)hub_dfs_kernel<true>, hub_lds_bytes(nlev, wmax));
+    static LdsLaunchCache no_lds;
+    return no_lds.get((const void *)hub_dfs_kernel<false>, 0);
 }
 
 }  // namespace beep

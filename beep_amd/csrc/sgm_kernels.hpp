@@ -46,6 +46,10 @@ struct KPlan {
     //   sum over x_b in B of C(|S n E[x_b]|, 2),
     // B = the candidates of b without the a constraints (hub + 4-cycle, W5).
     int32_t pair;
+    // Every row a level intersects with is used above its own index only
+    // (through the symmetry bounds): the bitmap builds set, of each adjacent
+    // pair, only the bit in the row of the smaller index.
+    int32_t upper;
     uint32_t qmask[kMaxLevels];
     uint32_t smask[kMaxLevels];
 };

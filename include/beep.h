@@ -120,6 +120,11 @@ typedef struct {
     uint32_t qedge[BEEP_MAXK * (BEEP_MAXK - 1) / 2];
     uint32_t sym_ptr[BEEP_MAXK + 1];
     uint32_t sym[BEEP_MAXK * (BEEP_MAXK - 1) / 2];
+    /* Every bitmap row a level intersects with is used above its own position
+     * only (through the symmetry bounds), so the bitmaps keep, of each adjacent
+     * pair, only the bit in the row of the smaller position (cliques, for
+     * instance). */
+    int upper_rows;
 } beep_plan_desc;
 
 int beep_plan_describe(const beep_plan *p, beep_plan_desc *out);
