@@ -38,6 +38,7 @@ struct DevGraph {
     int device = -1;
     uint32_t *ocol = nullptr;
     VInfo *vinfo = nullptr;
+    uint32_t *htab = nullptr;  // lookup tables of the larger-key ranges (2 m slots)
     ~DevGraph();
 };
 

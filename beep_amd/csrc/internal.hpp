@@ -196,22 +196,39 @@ void build_work(Prep &pp, const Plan &p, uint32_t sample_stride);
 inline uint32_t hub_sample_stride(uint32_t sample_stride) { return sample_stride >= 8 ? sample_stride / 8 : 1; }
 
 // LDS budget of the batch kernel (words / slots); shared with the kernels.
-// A batch of lane class lg holds roots of at most 64 << lg neighbours, so
-// its bitmaps take at most kMaxSumDeg << lg words.  The budget is sized so
-// that three to five workgroups share a CU (their memory latency overlaps).
+// A batch of lane class lg holds roots of at most 64 << lg neighbours and a
+// degree sum of at most sumdeg(lg), so its bitmaps take at most
+// sumdeg(lg) << lg words.  The budget is sized so that the narrow classes,
+// which carry most roots, fit six to seven workgroups on a CU (the batches
+// wait on dependent memory accesses; more of them in flight hide it); every
+// budget grows with lg, so the widest class's layout serves every batch.
 struct LdsBudget {
     static constexpr uint32_t kMaxRoots = 128;     // roots per batch
-    static constexpr uint32_t kMaxSumDeg = 512;    // sum of degrees per batch
+    static constexpr uint32_t kMaxSumDeg = 512;    // largest sum of degrees per batch
     static constexpr uint32_t kMaxEWords = 4096;   // bitmap words (u64)
-    static constexpr uint32_t kMaxHash = 2048;     // hash slots
+    static constexpr uint32_t kMaxHash = 2048;     // largest hash slots per batch
     static constexpr int kMaxClass = 6;
+    static constexpr uint32_t sumdeg(int lg) { return lg >= 3 ? kMaxSumDeg : kMaxSumDeg / 2; }
+    static constexpr uint32_t hash(int lg) { return 4 * sumdeg(lg); }
     static constexpr uint32_t ewords(int lg)
     {
-        return (kMaxSumDeg << (lg < 3 ? lg : 3)) < kMaxEWords ? (kMaxSumDeg << (lg < 3 ? lg : 3)) : kMaxEWords;
+        return (sumdeg(lg) << (lg < 3 ? lg : 3)) < kMaxEWords ? (sumdeg(lg) << (lg < 3 ? lg : 3)) : kMaxEWords;
+    }
+    // a root of d neighbours fits a batch of its lane class
+    static bool fits(uint32_t d, uint32_t hash_slots)
+    {
+        const uint32_t w = (d + 63) / 64;
+        int lg = 0;
+        while ((1u << lg) < w) lg++;
+        return (uint64_t)d * w <= ewords(lg) && d <= sumdeg(lg) && hash_slots <= hash(lg);
     }
 };
 
 uint32_t hash_slots_for(uint32_t d);
+// Automatic LDS bitmap build: a row vertex whose larger-key range is longer
+// than kLookupRatio x the root's degree looks the root's neighbours up in the
+// range's lookup table instead of streaming the range.
+constexpr uint32_t kLookupRatio = 4;
 void build_tiers(Prep &pp);
 void class_figures(Prep &pp);
 // Bytes of the kernels' per-root record of an LDS batch (RootRec).

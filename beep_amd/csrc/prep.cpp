@@ -289,11 +289,7 @@ void build_work(Prep &pp_, const Plan &p, uint32_t sample_stride)
             // length-sorted list) are kept every hub_sample_stride(), the rest
             // every sample_stride; the middle entry of every stride, so the
             // heaviest root alone does not stand for the ones after it.
-            auto fits_lds = [](uint32_t d) {
-                const uint64_t w = (d + 63) / 64;
-                return (uint64_t)d * w <= LdsBudget::kMaxEWords && d <= LdsBudget::kMaxSumDeg &&
-                       hash_slots_for(d) <= LdsBudget::kMaxHash;
-            };
+            auto fits_lds = [](uint32_t d) { return LdsBudget::fits(d, hash_slots_for(d)); };
             size_t nh = 0;
             while (nh < idx.size() && !fits_lds(len_of(idx[nh]))) nh++;
             const uint32_t hs = hub_sample_stride(sample_stride);
@@ -324,11 +320,7 @@ void build_work(Prep &pp_, const Plan &p, uint32_t sample_stride)
 void build_tiers(Prep &pp)
 {
     auto deg = [&](uint32_t i) { return pp.wlen[i]; };
-    auto fits_lds = [&](uint32_t d) {
-        uint64_t w = (d + 63) / 64;
-        return (uint64_t)d * w <= LdsBudget::kMaxEWords && d <= LdsBudget::kMaxSumDeg &&
-               hash_slots_for(d) <= LdsBudget::kMaxHash;
-    };
+    auto fits_lds = [&](uint32_t d) { return LdsBudget::fits(d, hash_slots_for(d)); };
     uint32_t nh = 0;
     while (nh < pp.wlen.size() && !fits_lds(deg(nh))) nh++;
     pp.n_hub = nh;
@@ -351,12 +343,12 @@ void build_tiers(Prep &pp)
         const uint32_t lg = lg_of(deg(i));
         uint64_t cap = class_sum[lg] / 1024;
         if (cap < 64) cap = 64;
-        if (cap > LdsBudget::kMaxSumDeg) cap = LdsBudget::kMaxSumDeg;
+        if (cap > LdsBudget::sumdeg((int)lg)) cap = LdsBudget::sumdeg((int)lg);
         while (i < nr && b.count < LdsBudget::kMaxRoots) {
             uint32_t d = deg(i);
             uint64_t w = (d + 63) / 64;
             if (b.count && (sd + d > cap || sw + d * w > LdsBudget::ewords((int)lg) ||
-                            sh + hash_slots_for(d) > LdsBudget::kMaxHash || lg_of(d) != lg))
+                            sh + hash_slots_for(d) > LdsBudget::hash((int)lg) || lg_of(d) != lg))
                 break;
             sd += d;
             sw += d * w;
@@ -371,9 +363,9 @@ void build_tiers(Prep &pp)
 }
 
 // Work and compulsory bytes of each lane class's launch, with the automatic
-// build mode (a row vertex whose scan range is over 32x the root's degree
-// looks the root's neighbours up by binary search, reading one entry and one
-// degree per step).
+// build mode (a row vertex whose scan range is over kLookupRatio x the root's
+// degree looks the root's neighbours on its larger-key side up in the range's
+// lookup table, one 4-byte slot each).
 void class_figures(Prep &pp)
 {
     const Variant &V = *pp.var;
@@ -392,11 +384,12 @@ void class_figures(Prep &pp)
                 const uint32_t v = V.ocol[rb + j];
                 const uint64_t hl = V.hlen[v];
                 bytes += 16;  // the row vertex's record
-                if (hl > 32ull * d) {
-                    uint64_t steps = 0;
-                    while ((1ull << steps) <= hl) steps++;
-                    units += d;
-                    bytes += 8ull * d * steps;
+                if (hl > (uint64_t)kLookupRatio * d) {
+                    // the root's neighbours on the larger-key side of v, one
+                    // table slot each
+                    const uint64_t c = V.asc ? d - 1 - j : j;
+                    units += c;
+                    bytes += 4ull * c;
                 } else {
                     units += hl;
                     bytes += 4ull * hl;

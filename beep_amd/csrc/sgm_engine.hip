@@ -61,6 +61,7 @@ DevGraph::~DevGraph()
     if (device >= 0) (void)hipSetDevice(device);
     if (ocol) (void)hipFree(ocol);
     if (vinfo) (void)hipFree(vinfo);
+    if (htab) (void)hipFree(htab);
 }
 
 Variant::~Variant() { delete dev; }
@@ -150,6 +151,25 @@ static void upload_variant(Variant &V, int device)
             }
         });
         dg->vinfo = dev_upload(vi);
+        {
+            // lookup tables: vertex v's larger-key range [hbeg, hbeg + hlen)
+            // in the 2 hlen slots at 2 hbeg, linear probing (KGraph::htab)
+            std::vector<uint32_t> ht(2 * V.m, 0u);
+            parallel_for(V.n, [&](size_t b, size_t e2) {
+                for (size_t v = b; v < e2; v++) {
+                    const uint64_t hb = vi[v].hbeg, hl = vi[v].hlen;
+                    const uint32_t size = (uint32_t)(2 * hl);
+                    uint32_t *t = ht.data() + 2 * hb;
+                    for (uint64_t x = 0; x < hl; x++) {
+                        const uint32_t w = V.ocol[hb + x];
+                        uint32_t s = lookup_slot(w, size);
+                        while (t[s]) s = s + 1 == size ? 0 : s + 1;
+                        t[s] = w + 1;
+                    }
+                }
+            }, 1024);
+            dg->htab = dev_upload(ht);
+        }
         progress("variant uploaded (%llu entries)", (unsigned long long)V.m);
     } catch (...) {
         delete dg;
@@ -417,7 +437,7 @@ static KPlan make_kplan(const Plan &p, int emit, int build_mode)
     k.L = p.L;
     k.skip_split = p.skip_split;
     k.emit = emit;
-    k.bin_ratio = build_mode == BEEP_BUILD_SCAN ? ~0u : build_mode == BEEP_BUILD_SEARCH ? 0u : 32u;
+    k.bin_ratio = build_mode == BEEP_BUILD_SCAN ? ~0u : build_mode == BEEP_BUILD_SEARCH ? 0u : kLookupRatio;
     k.tail = tail_summed(p, emit) ? 1 : 0;
     k.pair = pair_summed(p, emit) ? 1 : 0;
     k.upper = p.upper_rows;
@@ -443,7 +463,7 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     c.out = KOut{s.ctr + kCtrAcc, buf ? s.dbuf : nullptr, buf ? o.match_cap : 0, s.ctr + kCtrDeadline,
                  s.ctr + kCtrStop, s.prof};
     c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0, o.build_mode);
-    c.kg = KGraph{V.dev->ocol, V.dev->vinfo, V.asc};
+    c.kg = KGraph{V.dev->ocol, V.dev->vinfo, V.dev->htab, V.asc};
     c.nlev = p.L > 3 ? p.L - 3 : 1;
     c.hub_grid = hub_grid_of(R, c.nlev, cus);
     // donation ring (hub items with |C_2| > donate_min)

@@ -15,7 +15,9 @@
 //  * the bitmap is built by streaming, for every row vertex v, only the part
 //    of v's oriented row that holds neighbours with a larger (degree, id) key
 //    and probing the hash -- each adjacent pair inside N(u) is found once and
-//    both bits are set -- instead of d^2/2 binary searches;
+//    both bits are set -- instead of d^2/2 binary searches; when that part is
+//    long next to the root's degree, the root's neighbours on v's larger-key
+//    side are looked up in a hash table of it instead (one load each);
 //  * roots too large for LDS ("hubs") keep bitmap + hash in an HBM arena built
 //    by grid-wide kernels, and every wave of the grid takes (hub, x_1) items,
 //    so one hub is shared by the whole GPU.
@@ -261,20 +263,6 @@ __device__ __forceinline__ void emit_word(const KOut &out, const KPlan &pl, int 
     }
 }
 
-// (deg a, a) comes before (deg b, b) in an oriented row
-__device__ __forceinline__ bool row_before(bool asc, uint32_t da, uint32_t a, uint32_t db, uint32_t b)
-{
-    if (da != db) return asc ? da < db : da > db;
-    return a < b;
-}
-
-// a has a larger key than v, i.e. lies in v's larger-key range when adjacent
-__device__ __forceinline__ bool larger_key(bool asc, uint32_t da, uint32_t a, uint32_t dv, uint32_t v)
-{
-    if (da != dv) return da > dv;
-    return asc ? a > v : a < v;
-}
-
 __device__ __forceinline__ bool use_search(uint32_t hlen, uint32_t d, uint32_t ratio)
 {
     return (uint64_t)hlen > (uint64_t)d * ratio;
@@ -293,7 +281,7 @@ __host__ __device__ inline uint32_t align_up(uint32_t x, uint32_t a) { return (x
 // nlev + 3 >= L positions of x are kept per thread.
 __host__ inline LdsLayout make_lds_layout(int lg, uint32_t nlev)
 {
-    const uint32_t sd = LdsBudget::kMaxSumDeg, hs = LdsBudget::kMaxHash, ew = LdsBudget::ewords(lg);
+    const uint32_t sd = LdsBudget::sumdeg(lg), hs = LdsBudget::hash(lg), ew = LdsBudget::ewords(lg);
     LdsLayout L{};
     uint32_t o = 0;
     L.ewords = ew;
@@ -458,8 +446,10 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
         }
         const VInfo vi = rvi[q];
         sHBeg[j] = vi.hbeg | ((uint64_t)vi.hlen << kRangeShift);
-        // work units: the range's entries, or the root's neighbours to look up
-        sPre[j] = use_search(vi.hlen, rr.d, pl.bin_ratio) ? rr.d : vi.hlen;
+        // work units: the range's entries, or the root's neighbours on the
+        // larger-key side of the row (later in ascending rows, earlier in
+        // descending ones) to look up in the range's table
+        sPre[j] = use_search(vi.hlen, rr.d, pl.bin_ratio) ? (g.asc ? rr.d - 1 - jl : jl) : vi.hlen;
     }
     __syncthreads();
     if (needE) {
@@ -529,19 +519,21 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
                 seek(e);
                 const uint32_t off = e - j0;
                 if (srch) {
-                    // neighbour i of the root: is A[i] in A[j]'s larger-key range?
-                    const uint32_t i = off;
-                    if (i == jl) continue;
-                    const uint32_t a = sA[rr.doff + i], v = sA[j];
-                    const uint32_t da = g.vinfo[a].deg, dv = g.vinfo[v].deg;
-                    if (!larger_key(g.asc, da, a, dv, v)) continue;
-                    uint32_t blo = 0, bhi = hl;  // first entry not before a
-                    while (blo < bhi) {
-                        const uint32_t m = (blo + bhi) >> 1;
-                        const uint32_t x = g.ocol[hb + m];
-                        if (row_before(g.asc, g.vinfo[x].deg, x, da, a)) blo = m + 1; else bhi = m;
+                    // neighbour i of the root on the larger-key side of row j:
+                    // is A[i] in A[j]'s larger-key range?  (its lookup table)
+                    const uint32_t i = g.asc ? jl + 1 + off : off;
+                    const uint32_t a = sA[rr.doff + i] + 1u, size = 2 * hl;
+                    const uint32_t *t = g.htab + 2 * hb;
+                    uint32_t s = lookup_slot(a - 1u, size);
+                    for (;;) {
+                        const uint32_t x = t[s];
+                        if (x == a) {
+                            set_pair(sE + rr.eoff, W, jl, i, pl.upper);
+                            break;
+                        }
+                        if (x == 0u) break;
+                        s = s + 1 == size ? 0 : s + 1;
                     }
-                    if (blo < hl && g.ocol[hb + blo] == a) set_pair(sE + rr.eoff, W, jl, i, pl.upper);
                     continue;
                 }
                 const uint32_t w = ent[k];

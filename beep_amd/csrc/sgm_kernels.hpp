@@ -28,8 +28,9 @@ struct KPlan {
     int32_t skip_split;  // skip x_1 < split(u) (sgm_kernelsLD.cuh:60)
     int32_t emit;        // BEEP_EMIT_*
     // LDS bitmap build: a row vertex whose larger-key range is longer than
-    // bin_ratio x the root's degree looks the root's neighbours up in that
-    // range by binary search instead of streaming it (0: always, ~0u: never)
+    // bin_ratio x the root's degree looks the root's larger-key neighbours up
+    // in the range's lookup table instead of streaming the range (0: always,
+    // ~0u: never)
     uint32_t bin_ratio;
     // Count-only plans whose last level L-1 does not intersect with the row
     // of x_{L-2}: C_{L-1} depends on x_{L-2} only through the exclusion of
@@ -66,14 +67,23 @@ struct VInfo {
 struct KGraph {
     const uint32_t *ocol;    // degree-oriented rows
     const VInfo *vinfo;      // n records
+    // Lookup table of every vertex's larger-key range: the range [hbeg, hbeg +
+    // hlen) hashed into the 2 hlen slots at 2 hbeg (key w + 1, 0 empty,
+    // linear probing from lookup_slot(w, 2 hlen)).
+    const uint32_t *htab;
     int32_t asc;             // rows ascend by (degree, id); else degree descends, ids ascend
 };
 
+// First slot of w in a lookup table of `size` slots (host and device).
+__host__ __device__ __forceinline__ uint32_t lookup_slot(uint32_t w, uint32_t size)
+{
+    uint32_t h = w * 0x9E3779B1u;
+    h ^= h >> 15;
+    return (uint32_t)(((uint64_t)h * size) >> 32);
+}
+
 // Staged range of a row vertex: start (40 bits) and length (24 bits).
 constexpr int kRangeShift = 40;
-// LDS-batch row record: range start (bits 0-39) and flags -- the row looks the
-// root's neighbours up by binary search, or it has no work (one idle unit).
-constexpr int kRowSearch = 62, kRowIdle = 63;
 
 // Per-root record of an LDS batch, offsets relative to the batch.
 struct RootRec {

@@ -67,6 +67,8 @@ def parse():
                     help="batched: queries slower than this alone keep the default launch parameters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--large-batch-cap-ms", type=float, default=60000.0,
+                    help="large: run the completed queries again as one batch only below this total")
     ap.add_argument("--once", action="store_true",
                     help="profiling mode: one launch sequence per query, no warmup, no CPU leg")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -631,13 +633,17 @@ def run_large(args):
         rows.append(rec)
         print(f"[bench] {json.dumps(rec)}", file=sys.stderr, flush=True)
     batched = None
-    if len(ran) > 1:
+    plain_ms = sum(r["t_enum_ms"] for r in rows if "t_enum_ms" in r and not r["stopped"])
+    if len(ran) > 1 and plain_ms > args.large_batch_cap_ms:
+        batched = {"skipped": f"the completed queries take {plain_ms / 1000:.1f} s alone, above "
+                              f"--large-batch-cap-ms {args.large_batch_cap_ms:.0f}"}
+    elif len(ran) > 1:
         try:
             b = B.Batch(ran)
             res, t_first = b.run()
             res, t = b.run()
             batched = {"queries": [p.plan.k for p in ran], "ms": t, "first_launch_ms": t_first,
-                       "plain_ms": sum(r["t_enum_ms"] for r in rows if "t_enum_ms" in r and not r["stopped"]),
+                       "plain_ms": plain_ms,
                        "counts": [r["count"] for r in res]}
         except B.BeepError as e:
             batched = {"error": str(e)}
