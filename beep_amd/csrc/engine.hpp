@@ -143,7 +143,8 @@ struct LaunchCfg {
 // Zero fill kernel (no memset nodes, so sequences can be captured) and the
 // first kernel of a launch sequence, which clears n counter words and writes
 // the deadlines at words dl, dl + 1.
-hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st);
+// dl: skip the clear once that deadline has passed (hub arenas only)
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st, const unsigned long long *dl = nullptr);
 hipError_t launch_begin_sequence(unsigned long long *ctr, uint64_t n, uint64_t dl, uint64_t ticks_hub,
                                  uint64_t ticks_lds, hipStream_t st);
 

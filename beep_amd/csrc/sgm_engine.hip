@@ -27,7 +27,8 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
                               uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
                               int *grid_out);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv);
+                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv,
+                           const unsigned long long *dl);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                              const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
                              const uint64_t *kv, uint64_t *E, int upper, const KOut &out);
@@ -506,10 +507,10 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
         const HubRound &r = R.rounds[ri];
         const uint64_t rows = r.g1 - r.g0;
         if (pp.uses_bitmap) {
-            HIP_CHECK(launch_fill_zero(s.arena, r.ewords * 8, st));
-            HIP_CHECK(launch_fill_zero(s.kv, r.hslots * 8, st));
+            HIP_CHECK(launch_fill_zero(s.arena, r.ewords * 8, st, c.out.dl));
+            HIP_CHECK(launch_fill_zero(s.kv, r.hslots * 8, st, c.out.dl));
             int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)cus * 16);
-            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv));
+            HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv, c.out.dl));
             int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
             HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, R.hub_range, r.g0, r.g1, s.kv,
                                         s.arena, c.kp.upper, c.out));

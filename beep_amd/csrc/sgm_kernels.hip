@@ -829,8 +829,10 @@ __global__ __launch_bounds__(kBlock) void lds_multi_kernel(const MultiQuery *__r
 // Hash of every hub row: vertex -> row index (linear probing, 0 = empty).
 __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec *__restrict__ hubs,
                                                           const uint32_t *__restrict__ hub_of_row,
-                                                          uint64_t g0, uint64_t g1, uint64_t *kv)
+                                                          uint64_t g0, uint64_t g1, uint64_t *kv,
+                                                          const unsigned long long *dl)
 {
+    if (past(*dl)) return;  // the encode and the DFS that follow stop at once
     for (uint64_t r = g0 + blockIdx.x * (uint64_t)kBlock + threadIdx.x; r < g1;
          r += (uint64_t)gridDim.x * kBlock) {
         const HubRec h = hubs[hub_of_row[r]];
@@ -1535,19 +1537,22 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
 }
 
 // Zero fill (used instead of memset nodes so launch sequences can be captured).
-__global__ void fill_zero_u32_kernel(uint32_t *p, uint64_t n)
+// With dl set, a workgroup that starts after that deadline clears nothing: a
+// hub round's arena is only read by kernels that stop at the same deadline.
+__global__ void fill_zero_u32_kernel(uint32_t *p, uint64_t n, const unsigned long long *dl)
 {
+    if (dl && past(*dl)) return;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = 0u;
 }
 
-hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st)
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t st, const unsigned long long *dl)
 {
     const uint64_t n = (bytes + 3) / 4;
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(fill_zero_u32_kernel, dim3((uint32_t)blocks), dim3(kBlock), 0, st, (uint32_t *)p, n);
+    hipLaunchKernelGGL(fill_zero_u32_kernel, dim3((uint32_t)blocks), dim3(kBlock), 0, st, (uint32_t *)p, n, dl);
     return hipGetLastError();
 }
 
@@ -1594,9 +1599,10 @@ hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const
 }
 
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv)
+                           const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv,
+                           const unsigned long long *dl)
 {
-    hipLaunchKernelGGL(hub_hash_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, kv);
+    hipLaunchKernelGGL(hub_hash_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, g0, g1, kv, dl);
     return hipGetLastError();
 }
 
