@@ -25,7 +25,7 @@ static_assert(sizeof(RootRec) == kRootRecBytes, "per-root record size");
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
                               uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
-                              int *grid_out);
+                              int *grid_out, bool dense = false);
 hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv,
                            const unsigned long long *dl);
@@ -653,8 +653,10 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
         hipStream_t sk = k == 0 ? st : side_stream(k);
         const int i = lane.ncls++;
         unsigned long long *cur = s.ctr + kCtrLds + (uint64_t)order[n] * kCurWords;
+        // without hub kernels beside them, the classes take the registers
+        // for 7 workgroups per CU
         HIP_CHECK(launch_lds_batches(lc.lg, pp.dev->cus, sk, c.kg, c.kp, R.recs, R.batch_first, R.batch_count,
-                                     lc.b0, lc.b1, cur, c.out, nullptr));
+                                     lc.b0, lc.b1, cur, c.out, nullptr, !hubs));
         HIP_CHECK(hipEventRecord(lane.cls_done[i], sk));
         lane.cls_lg[i] = (uint8_t)lc.lg;
         lane.cls_after[i] = (int8_t)last[k];

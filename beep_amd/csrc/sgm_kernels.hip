@@ -743,8 +743,11 @@ __device__ void process_batch(const KGraph &g, const KPlan &pl, const LdsView &v
     __syncthreads();
 }
 
-template <int P>
-__global__ __launch_bounds__(kBlock) void lds_batch_kernel(KGraph g, KPlan pl,
+// MinWaves > 1: the compiler keeps the registers to that many waves per SIMD
+// (7 workgroups per CU instead of 5 for the narrow classes, at the cost of a
+// few bytes of spill); used when no hub kernels of the query share the CUs.
+template <int P, int MinWaves>
+__global__ __launch_bounds__(kBlock, MinWaves) void lds_batch_kernel(KGraph g, KPlan pl,
                                                            const RootRec *__restrict__ recs,
                                                            const uint32_t *__restrict__ batch_first,
                                                            const uint32_t *__restrict__ batch_count,
@@ -1457,14 +1460,14 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
 
 // ------------------------------------------------------------ launchers
 
-template <int P>
+template <int P, int MinWaves>
 static hipError_t launch_lds_p(int grid, size_t lds, hipStream_t st, const KGraph &g, const KPlan &pl,
                                const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
                                uint32_t b0, uint32_t b1, unsigned long long *cursor, const LdsLayout &lay,
                                const KOut &out)
 {
-    hipLaunchKernelGGL(lds_batch_kernel<P>, dim3(grid), dim3(kBlock), lds, st, g, pl, recs, bf, bc, b0,
-                       b1, cursor, lay, out);
+    hipLaunchKernelGGL((lds_batch_kernel<P, MinWaves>), dim3(grid), dim3(kBlock), lds, st, g, pl, recs, bf, bc,
+                       b0, b1, cursor, lay, out);
     return hipGetLastError();
 }
 
@@ -1497,27 +1500,27 @@ struct LdsLaunchCache {
     }
 };
 
-template <int P>
+template <int P, int MinWaves>
 static int occupancy_lds_p(size_t lds)
 {
     static LdsLaunchCache cache;
-    return cache.get((const void *)lds_batch_kernel<P>, lds);
+    return cache.get((const void *)lds_batch_kernel<P, MinWaves>, lds);
 }
 
 // Dispatch on the DFS group width.
 hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, const KPlan &pl,
                               const RootRec *recs, const uint32_t *bf, const uint32_t *bc,
                               uint32_t b0, uint32_t b1, unsigned long long *cursor, const KOut &out,
-                              int *grid_out)
+                              int *grid_out, bool dense)
 {
     const uint32_t nlev = pl.L > 3 ? (uint32_t)(pl.L - 3) : 1u;
     LdsLayout lay = make_lds_layout(lg, nlev);
     size_t lds = lay.bytes;
     int occ = 1;
     switch (lg) {
-#define BEEP_CASE(LG)                                   \
-    case LG:                                            \
-        occ = occupancy_lds_p<(1 << LG)>(lds);          \
+#define BEEP_CASE(LG)                                                                              \
+    case LG:                                                                                       \
+        occ = dense ? occupancy_lds_p<(1 << LG), 7>(lds) : occupancy_lds_p<(1 << LG), 1>(lds);    \
         break;
         BEEP_CASE(0) BEEP_CASE(1) BEEP_CASE(2) BEEP_CASE(3) BEEP_CASE(4) BEEP_CASE(5) BEEP_CASE(6)
 #undef BEEP_CASE
@@ -1528,8 +1531,10 @@ hipError_t launch_lds_batches(int lg, int cus, hipStream_t st, const KGraph &g, 
     if (grid < 1) grid = 1;
     if (grid_out) *grid_out = grid;
     switch (lg) {
-#define BEEP_CASE(LG) \
-    case LG: return launch_lds_p<(1 << LG)>(grid, lds, st, g, pl, recs, bf, bc, b0, b1, cursor, lay, out);
+#define BEEP_CASE(LG)                                                                                    \
+    case LG:                                                                                             \
+        return dense ? launch_lds_p<(1 << LG), 7>(grid, lds, st, g, pl, recs, bf, bc, b0, b1, cursor, lay, out) \
+                     : launch_lds_p<(1 << LG), 1>(grid, lds, st, g, pl, recs, bf, bc, b0, b1, cursor, lay, out);
         BEEP_CASE(0) BEEP_CASE(1) BEEP_CASE(2) BEEP_CASE(3) BEEP_CASE(4) BEEP_CASE(5) BEEP_CASE(6)
 #undef BEEP_CASE
     }

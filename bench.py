@@ -300,7 +300,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL,
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
-                     "traffic": dom_traffic,
+                     "traffic": dom_traffic / max(len(preps), 1) if dom_traffic else None,
                      "alg_bytes_per_launch": dom_bytes / max(len(preps), 1),
                      "launches_per_step": len(preps),
                      "kernel_ms_per_step": dom_ms,
