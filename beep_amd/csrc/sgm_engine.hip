@@ -803,7 +803,11 @@ struct Batch {
     MultiWork *dwork = nullptr;
     uint32_t nwork = 0;
     uint32_t nlev = 1;
-    unsigned long long *cursor = nullptr;  // striped
+    // lane classes: work records [cls_w0, cls_w1) of width cls_lg, one launch
+    // and one striped cursor (kCurWords words) each
+    std::vector<int> cls_lg;
+    std::vector<uint32_t> cls_w0, cls_w1;
+    unsigned long long *cursor = nullptr;
     hipStream_t s0 = nullptr;
     std::vector<hipStream_t> branch;
     std::vector<hipEvent_t> done;
@@ -830,7 +834,7 @@ struct Batch {
 };
 
 hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work, uint32_t nwork,
-                            unsigned long long *cursor, uint32_t nlev, int *grid_out);
+                            unsigned long long *cursor, uint32_t nlev, int lg, int *grid_out);
 
 Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opts *opts, int n)
 {
@@ -871,17 +875,30 @@ Batch *batch_create(Prep *const *preps, const Plan *const *plans, const beep_opt
                 work.push_back({cost, MultiWork{(uint32_t)i, bi, lg, 0}});
             }
         }
-        std::stable_sort(work.begin(), work.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+        // by lane class, heaviest first within a class
+        std::stable_sort(work.begin(), work.end(), [](const auto &a, const auto &c) {
+            if (a.second.lanes_log2 != c.second.lanes_log2) return a.second.lanes_log2 > c.second.lanes_log2;
+            return a.first > c.first;
+        });
         std::vector<MultiWork> w;
-        for (auto &x : work) w.push_back(x.second);
+        for (auto &x : work) {
+            const int lg = (int)x.second.lanes_log2;
+            if (b->cls_lg.empty() || b->cls_lg.back() != lg) {
+                b->cls_lg.push_back(lg);
+                b->cls_w0.push_back((uint32_t)w.size());
+                b->cls_w1.push_back((uint32_t)w.size());
+            }
+            w.push_back(x.second);
+            b->cls_w1.back() = (uint32_t)w.size();
+        }
         HIP_CHECK(hipSetDevice(b->device));
         b->dqs = dev_upload(qs);
         b->dwork = dev_upload(w);
         b->nwork = (uint32_t)w.size();
-        b->cursor = dev_alloc<unsigned long long>(kCurWords);
+        b->cursor = dev_alloc<unsigned long long>(kCurWords * std::max<size_t>(1, b->cls_lg.size()));
         HIP_CHECK(hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking));
-        b->branch.resize(n + 1);
-        b->done.resize(n + 1);
+        b->branch.resize(n + b->cls_lg.size() + 1);
+        b->done.resize(n + b->cls_lg.size() + 1);
         for (auto &s : b->branch) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         for (auto &e : b->done) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
@@ -904,21 +921,27 @@ static void batch_capture(Batch &b)
                                         b.cfgs[i].deadline_ticks_lds, b.s0));
         if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, b.s0));
     }
-    HIP_CHECK(launch_fill_zero(b.cursor, kCurWords * 8, b.s0));
-    // The persistent LDS kernel runs first, on the whole GPU; the hub branches
-    // follow it concurrently.  A persistent hub DFS dispatched while the LDS
-    // kernel still holds every CU's LDS stays packed onto the CUs it got
-    // (rmat22cut's W5 drain ran 48 s beside it instead of 36 s alone).
+    const int ncls = (int)b.cls_lg.size();
+    HIP_CHECK(launch_fill_zero(b.cursor, (uint64_t)kCurWords * 8 * std::max(ncls, 1), b.s0));
+    // Every query's hub rounds on a branch of its own, then one persistent
+    // launch per lane class (each with that class's LDS layout, so the narrow
+    // classes keep their occupancy), widest first, each on a branch: they all
+    // share the GPU, as the classes and hub rounds of one query do.
     const DevPrep &d0 = *b.preps[0]->dev;
-    if (b.nwork)
-        HIP_CHECK(launch_lds_multi(d0.cus, b.s0, b.dqs, b.dwork, b.nwork, b.cursor, b.nlev, nullptr));
     HIP_CHECK(hipEventRecord(b.fork, b.s0));
     for (int i = 0; i < n; i++) {
         HIP_CHECK(hipStreamWaitEvent(b.branch[i], b.fork, 0));
         enqueue_hubs(*b.preps[i], b.cfgs[i], b.scr[i], b.branch[i]);
         HIP_CHECK(hipEventRecord(b.done[i], b.branch[i]));
     }
-    for (int i = 0; i < n; i++) HIP_CHECK(hipStreamWaitEvent(b.s0, b.done[i], 0));
+    for (int k = 0; k < ncls; k++) {
+        hipStream_t sk = b.branch[n + k];
+        HIP_CHECK(hipStreamWaitEvent(sk, b.fork, 0));
+        HIP_CHECK(launch_lds_multi(d0.cus, sk, b.dqs, b.dwork + b.cls_w0[k], b.cls_w1[k] - b.cls_w0[k],
+                                   b.cursor + (uint64_t)k * kCurWords, b.nlev, b.cls_lg[k], nullptr));
+        HIP_CHECK(hipEventRecord(b.done[n + k], sk));
+    }
+    for (int i = 0; i < n + ncls; i++) HIP_CHECK(hipStreamWaitEvent(b.s0, b.done[i], 0));
     HIP_CHECK(hipStreamEndCapture(b.s0, &b.graph));
     HIP_CHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
 }

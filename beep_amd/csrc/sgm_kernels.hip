@@ -1158,7 +1158,11 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     // x_1..x_{l-2} fixed; B (level b without its constraints on x_a = x_{l-1})
     // goes to the next stack row, then C(|S n E[x_b]|, 2) is added for every
     // x_b in B.
-    auto pair_count = [&](int l) {
+    // Returns true when the deadline passed before it was done (a hub of
+    // hundreds of thousands of neighbours makes one call long): the item is
+    // left with part of its count.
+    auto pair_count = [&](int l) -> bool {
+        uint32_t steps = 0;
         const uint64_t *S = my + (uint64_t)(l - 3) * wmax;
         uint64_t *Bv = my + (uint64_t)(l - 2) * wmax;
         const uint32_t blo = word_lo(l);
@@ -1185,6 +1189,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 }
             }
             if (xs == 0xffffffffu) break;
+            if ((++steps & 63u) == 0u && past(dl)) return true;
             const uint64_t *Es = Eh + (uint64_t)(xs - rlo) * W;  // x_a's row: kept, as the walk reads it
             for (uint32_t rr = blo >> 6; rr < R; rr++) {
                 const uint32_t w = rr * 64 + lane;
@@ -1234,6 +1239,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 }
             }
             if (xb == 0xffffffffu) break;
+            if ((++steps & 63u) == 0u && past(dl)) return true;
             const uint64_t *Eb = Eh + (uint64_t)(xb - rlo) * W;
             uint32_t pc = 0;
             for (uint32_t rr = slo >> 6; rr < R; rr++) {
@@ -1247,6 +1253,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             if (lane == 0) cnt += (uint64_t)pc * (pc - 1u) / 2u;
             pos = xb + 1u;
         }
+        return false;
     };
 
     // DFS below level 2 with C_2 in my[0, W): x_2 runs over C_2 n [pos0, hi2).
@@ -1266,7 +1273,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 continue;
             }
             if (pl.pair && l == L - 2) {  // levels L-3..L-1 at once (never split by donation at l = 3)
-                pair_count(l);
+                if (pair_count(l)) break;  // the next claim records the stop
                 if (l == 3) break;
                 --l;
                 pos = xget(l - 1) + 1u;
@@ -1586,15 +1593,14 @@ hipError_t launch_begin_sequence(unsigned long long *ctr, uint64_t n, uint64_t d
     return hipGetLastError();
 }
 
+// One lane class's batches (lg: the widest of them) of several queries.
 hipError_t launch_lds_multi(int cus, hipStream_t st, const MultiQuery *qs, const MultiWork *work,
-                            uint32_t nwork, unsigned long long *cursor, uint32_t nlev, int *grid_out)
+                            uint32_t nwork, unsigned long long *cursor, uint32_t nlev, int lg, int *grid_out)
 {
-    LdsLayout lay = make_lds_layout(LdsBudget::kMaxClass, nlev);
+    LdsLayout lay = make_lds_layout(lg, nlev);
     size_t lds = lay.bytes;
-    (void)hipFuncSetAttribute((const void *)lds_multi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, lds_multi_kernel, kBlock, lds) != hipSuccess || occ < 1)
-        occ = 1;
+    static LdsLaunchCache cache;
+    const int occ = cache.get((const void *)lds_multi_kernel, lds);
     int grid = occ * cus;
     if ((uint32_t)grid > nwork) grid = (int)nwork;
     if (grid < 1) grid = 1;

@@ -67,6 +67,8 @@ def parse():
                     help="batched: queries slower than this alone keep the default launch parameters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--large-queries", default="",
+                    help="large: comma-separated subset of the workload's queries (default: all)")
     ap.add_argument("--large-batch-cap-ms", type=float, default=60000.0,
                     help="large: run the completed queries again as one batch only below this total")
     ap.add_argument("--once", action="store_true",
@@ -136,7 +138,7 @@ def host_cpu():
 # roots with at most 64 row entries (lane class 0, one DFS lane per item) -- the
 # largest share of device time in the kernel trace of this workload.
 DOMINANT_LG = 0
-DOMINANT_KERNEL = "lds_batch_kernel<1>"
+DOMINANT_KERNEL = "lds_batch_kernel<1, *> (narrowest lane class)"
 
 
 def main():
@@ -605,7 +607,10 @@ def run_large(args):
           file=sys.stderr, flush=True)
     dl = args.deadline_ms if args.deadline_ms > 0 else 0.0
     rows, ran = [], []
+    pick = [x for x in args.large_queries.split(",") if x]
     for q, task in wl["queries"]:
+        if pick and q not in pick:
+            continue
         s, d = P.directed(P.read_mtx_edges(q))
         rec = {"query": q, "mode": task}
         try:
