@@ -48,6 +48,7 @@ struct HubRound {
     uint64_t i0, i1;      // global item range (DFS)
     uint64_t ewords;      // bitmap words used
     uint64_t hslots;      // hash slots used
+    uint64_t er0 = 0, er1 = 0;  // its rows with a non-empty scan range: enc_rows[er0, er1)
 };
 
 // Read-only device records of one prepared root queue.
@@ -60,6 +61,7 @@ struct DevRoots {
     // per hub row: the part of the row vertex's larger-key range the bitmap
     // build streams (start | length << kRangeShift)
     uint64_t *hub_range = nullptr;
+    uint64_t *enc_rows = nullptr;  // per round, the hub rows whose scan range is not empty
     int skip_split = -1;  // plan property the hub items were built for
     uint32_t nhubs = 0;
     uint32_t wmax = 0;
@@ -102,12 +104,12 @@ struct Lane {
     hipStream_t side[kMaxSide] = {};
     hipEvent_t ev[4] = {};  // start, hubs done, all done, counters copied
     hipEvent_t side_done[kMaxSide] = {};
-    // end of each LDS lane class's launch, its lane class, and what it started
-    // after (-1: ev[0], -2: ev[1], i >= 0: cls_done[i])
+    // start and end of each LDS lane class's launch (recorded on its stream
+    // right before and after it) and its lane class
     static constexpr int kMaxClasses = 32;
+    hipEvent_t cls_start[kMaxClasses] = {};
     hipEvent_t cls_done[kMaxClasses] = {};
     uint8_t cls_lg[kMaxClasses] = {};
-    int8_t cls_after[kMaxClasses] = {};
     int ncls = 0;
     void create(int device, int nside);
     ~Lane();

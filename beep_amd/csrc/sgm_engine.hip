@@ -30,7 +30,7 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
                            const uint32_t *hor, uint64_t g0, uint64_t g1, uint64_t *kv,
                            const unsigned long long *dl);
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                             const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
+                             const uint32_t *hor, const uint64_t *hrange, const uint64_t *rows, uint64_t nrows,
                              const uint64_t *kv, uint64_t *E, int upper, const KOut &out);
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
@@ -70,12 +70,12 @@ Variant::~Variant() { delete dev; }
 void DevRoots::release()
 {
     for (void *p : {(void *)recs, (void *)batch_first, (void *)batch_count, (void *)hubs, (void *)hub_of_row,
-                    (void *)hub_of_item, (void *)hub_range})
+                    (void *)hub_of_item, (void *)hub_range, (void *)enc_rows})
         if (p) (void)hipFree(p);
     recs = nullptr;
     batch_first = batch_count = hub_of_row = hub_of_item = nullptr;
     hubs = nullptr;
-    hub_range = nullptr;
+    hub_range = enc_rows = nullptr;
 }
 
 void Lane::create(int dev, int ns)
@@ -88,6 +88,7 @@ void Lane::create(int dev, int ns)
     for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
     for (int i = 0; i < nside; i++) HIP_CHECK(hipEventCreate(&side_done[i]));
     for (auto &e : cls_done) HIP_CHECK(hipEventCreate(&e));
+    for (auto &e : cls_start) HIP_CHECK(hipEventCreate(&e));
 }
 
 Lane::~Lane()
@@ -96,6 +97,8 @@ Lane::~Lane()
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : side_done)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : cls_start)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : cls_done)
         if (e) (void)hipEventDestroy(e);
@@ -335,7 +338,20 @@ void upload(Prep &pp, int device, int p_skip_split, uint64_t arena_budget)
             R.arena_words = std::max(R.arena_words, r.ewords);
             R.arena_slots = std::max(R.arena_slots, r.hslots);
         }
-        if (pp.uses_bitmap) R.hub_range = dev_upload(hub_scan_ranges(V, hubs, pp.upper_rows != 0));
+        if (pp.uses_bitmap) {
+            const std::vector<uint64_t> ranges = hub_scan_ranges(V, hubs, pp.upper_rows != 0);
+            // the bitmap build streams only rows with a non-empty range (a
+            // third of the diamond's hub rows on chunglu1m have none)
+            std::vector<uint64_t> rows;
+            for (auto &r : R.rounds) {
+                r.er0 = rows.size();
+                for (uint64_t g = r.g0; g < r.g1; g++)
+                    if (ranges[g] >> kRangeShift) rows.push_back(g);
+                r.er1 = rows.size();
+            }
+            if (!rows.empty()) R.enc_rows = dev_upload(rows);
+            R.hub_range = dev_upload(ranges);
+        }
         R.hubs = dev_upload(hubs);
         R.hub_of_row = dev_upload(hor);
         R.hub_of_item = dev_upload(hoi);
@@ -511,9 +527,11 @@ void enqueue_hubs(const Prep &pp, const LaunchCfg &c, const Scratch &s, hipStrea
             HIP_CHECK(launch_fill_zero(s.kv, r.hslots * 8, st, c.out.dl));
             int g1 = (int)std::min<uint64_t>((rows + kBlock - 1) / kBlock, (uint64_t)cus * 16);
             HIP_CHECK(launch_hub_hash(std::max(g1, 1), st, c.kg, R.hubs, R.hub_of_row, r.g0, r.g1, s.kv, c.out.dl));
-            int g2 = (int)std::min<uint64_t>((rows + 255) / 256, (uint64_t)cus * 16);
-            HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, R.hub_range, r.g0, r.g1, s.kv,
-                                        s.arena, c.kp.upper, c.out));
+            const uint64_t nenc = r.er1 - r.er0;
+            int g2 = (int)std::min<uint64_t>((nenc + 255) / 256, (uint64_t)cus * 16);
+            if (nenc)
+                HIP_CHECK(launch_hub_encode(std::max(g2, 1), st, c.kg, R.hubs, R.hub_of_row, R.hub_range,
+                                            R.enc_rows + r.er0, nenc, s.kv, s.arena, c.kp.upper, c.out));
         }
         unsigned long long *cur = s.ctr + kCtrHub + (uint64_t)(ri % 64) * kCurWords;
         if (ri >= 64) HIP_CHECK(launch_fill_zero(cur, kCurWords * 8, st));
@@ -563,10 +581,8 @@ void class_times(const Lane &lane, double *t_by_lg)
 {
     for (int i = 0; i < 8; i++) t_by_lg[i] = 0;
     for (int i = 0; i < lane.ncls; i++) {
-        const int a = lane.cls_after[i];
-        const hipEvent_t start = a == -1 ? lane.ev[0] : a == -2 ? lane.ev[1] : lane.cls_done[a];
         float t = 0;
-        HIP_CHECK(hipEventElapsedTime(&t, start, lane.cls_done[i]));
+        HIP_CHECK(hipEventElapsedTime(&t, lane.cls_start[i], lane.cls_done[i]));
         if (lane.cls_lg[i] < 8) t_by_lg[lane.cls_lg[i]] += t;
     }
 }
@@ -614,26 +630,28 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
     HIP_CHECK(launch_begin_sequence(s.ctr, kCtrTotal, kCtrDeadline, c.deadline_ticks, c.deadline_ticks_lds, st));
     if (s.prof) HIP_CHECK(launch_fill_zero(s.prof, s.cap.prof_words * 8, st));
     HIP_CHECK(hipEventRecord(lane.ev[0], st));
-    // The heaviest LDS lane class starts on the main stream at once (a wait on
-    // another stream's event costs tens of microseconds); the hub rounds, the
-    // longest chain of dependent kernels, are queued next, on side[0]; the
-    // other classes follow widest first (their batches take longest, so they
-    // start early and the narrow batches fill in around them), one on each
-    // further side stream.  Classes left over queue on main behind the first
-    // one, which holds the CUs' LDS while it runs: they start as it drains.
-    // Without hubs, side[0] takes a class too.
+    // With hub roots, the heaviest LDS lane class starts on the main stream at
+    // once (a wait on another stream's event costs tens of microseconds); the
+    // hub rounds, the longest chain of dependent kernels, are queued next, on
+    // side[0]; the other classes follow widest first (their batches take
+    // longest, so they start early and the narrow batches fill in around
+    // them), one on each further side stream.  Classes left over queue on main
+    // behind the first one, which holds the CUs' LDS while it runs: they start
+    // as it drains.  Without hubs, the widest class takes the main stream and
+    // the others side[0], side[1], ...
     std::vector<LdsClass> cls = c.kp.L >= 3 ? lds_classes(pp, c) : std::vector<LdsClass>();
     if (cls.size() > (size_t)Lane::kMaxClasses) throw Error(BEEP_ERANGE, "too many lane classes");
     std::vector<int> order(cls.size());
     for (size_t i = 0; i < cls.size(); i++) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a].work > cls[b].work; });
-    if (order.size() > 2)
-        std::stable_sort(order.begin() + 1, order.end(), [&](int a, int b) { return cls[a].lg > cls[b].lg; });
     const DevRoots &R = pp.dev->roots;
     const bool hubs = c.kp.L >= 3 && !R.rounds.empty();
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a].work > cls[b].work; });
+    if (!hubs)  // widest first, on main; the narrow classes fill in around it
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a].lg > cls[b].lg; });
+    else if (order.size() > 2)
+        std::stable_sort(order.begin() + 1, order.end(), [&](int a, int b) { return cls[a].lg > cls[b].lg; });
     // stream k: 0 = main, 1 + i = side[i]
     const int nstreams = 1 + lane.nside;
-    std::vector<int> last(nstreams, -1);  // what the next launch on stream k starts after
     std::vector<bool> waited(nstreams, false);
     waited[0] = true;
     auto side_stream = [&](int k) -> hipStream_t {
@@ -655,12 +673,11 @@ void enqueue_query(const Prep &pp, const LaunchCfg &c, const Scratch &s, Lane &l
         unsigned long long *cur = s.ctr + kCtrLds + (uint64_t)order[n] * kCurWords;
         // without hub kernels beside them, the classes take the registers
         // for 7 workgroups per CU
+        HIP_CHECK(hipEventRecord(lane.cls_start[i], sk));
         HIP_CHECK(launch_lds_batches(lc.lg, pp.dev->cus, sk, c.kg, c.kp, R.recs, R.batch_first, R.batch_count,
                                      lc.b0, lc.b1, cur, c.out, nullptr, !hubs));
         HIP_CHECK(hipEventRecord(lane.cls_done[i], sk));
         lane.cls_lg[i] = (uint8_t)lc.lg;
-        lane.cls_after[i] = (int8_t)last[k];
-        last[k] = i;
         if (n == 0) {
             hs = side_stream(1);
             if (hubs) enqueue_hubs(pp, c, s, hs);

@@ -855,7 +855,8 @@ __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec
     }
 }
 
-// Hub bitmaps: waves take chunks of 64 rows, scan the lengths of their scan
+// Hub bitmaps: waves take chunks of 64 rows (of the round's rows whose scan
+// range is not empty, listed on the host), scan the lengths of their scan
 // ranges (the larger-key range of the row vertex, cut on the host to the part
 // that can hold kept columns) and stream the concatenation lane-parallel.  A
 // chunk takes every nchunks-th row, so the long ranges of one hub's rows
@@ -864,7 +865,7 @@ __global__ __launch_bounds__(kBlock) void hub_hash_kernel(KGraph g, const HubRec
 __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubRec *__restrict__ hubs,
                                                             const uint32_t *__restrict__ hub_of_row,
                                                             const uint64_t *__restrict__ hub_range,
-                                                            uint64_t g0, uint64_t g1,
+                                                            const uint64_t *__restrict__ rows, uint64_t nrows,
                                                             const uint64_t *__restrict__ kv,
                                                             uint64_t *E, int upper, KOut out)
 {
@@ -873,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
     __shared__ uint32_t sHub[kBlock];
     __shared__ uint32_t sJl[kBlock];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const uint64_t nchunks = (g1 - g0 + 63) / 64;
+    const uint64_t nchunks = (nrows + 63) / 64;
     const uint64_t wstride = (uint64_t)gridDim.x * (kBlock / 64);
     const uint64_t dl = *out.dl;
     for (uint64_t c = blockIdx.x * (uint64_t)(kBlock / 64) + wv; c < nchunks; c += wstride) {
@@ -881,9 +882,10 @@ __global__ __launch_bounds__(kBlock) void hub_encode_kernel(KGraph g, const HubR
             if (lane == 0) mark_stop(out);
             break;
         }
-        const uint64_t r = g0 + (uint64_t)lane * nchunks + c;  // rows strided over the chunks
+        const uint64_t x = (uint64_t)lane * nchunks + c;  // rows strided over the chunks
         uint32_t len = 0;
-        if (r < g1) {
+        if (x < nrows) {
+            const uint64_t r = rows[x];
             const uint32_t hi = hub_of_row[r];
             const uint64_t rg = hub_range[r];
             sBeg[tid] = rg & ((1ull << kRangeShift) - 1ull);
@@ -1618,10 +1620,10 @@ hipError_t launch_hub_hash(int grid, hipStream_t st, const KGraph &g, const HubR
 }
 
 hipError_t launch_hub_encode(int grid, hipStream_t st, const KGraph &g, const HubRec *hubs,
-                             const uint32_t *hor, const uint64_t *hrange, uint64_t g0, uint64_t g1,
+                             const uint32_t *hor, const uint64_t *hrange, const uint64_t *rows, uint64_t nrows,
                              const uint64_t *kv, uint64_t *E, int upper, const KOut &out)
 {
-    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, hrange, g0, g1, kv, E,
+    hipLaunchKernelGGL(hub_encode_kernel, dim3(grid), dim3(kBlock), 0, st, g, hubs, hor, hrange, rows, nrows, kv, E,
                        upper, out);
     return hipGetLastError();
 }
