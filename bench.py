@@ -8,8 +8,9 @@ sequence in sgm mode (counting, no tuple output -- the reference's sgm).
 
 Reported:
   value        matches/s = (count_4clique + count_diamond) / device time
-  roofline     the dominant kernel, lds_batch_kernel<1> (LDS batches of roots
-               with at most 64 row entries): the bytes its launches must read
+  roofline     the dominant kernel, lds_batch_kernel<1, *> (LDS batches of
+               roots with at most 64 row entries; its two occupancy
+               instantiations are one kernel here): the bytes its launches must read
                at least once (root records and rows, the row vertices' records
                and bitmap-build scan ranges; computed on the host per lane
                class) over its device time, measured with HIP events on the
@@ -134,8 +135,8 @@ def host_cpu():
     return model, os.cpu_count() or 1, usable
 
 
-# The kernel the roofline line is about: lds_batch_kernel<1>, the LDS batches of
-# roots with at most 64 row entries (lane class 0, one DFS lane per item) -- the
+# The kernel the roofline line is about: lds_batch_kernel<1, *>, the LDS batches
+# of roots with at most 64 row entries (lane class 0, one DFS lane per item) -- the
 # largest share of device time in the kernel trace of this workload.
 DOMINANT_LG = 0
 DOMINANT_KERNEL = "lds_batch_kernel<1, *> (narrowest lane class)"
@@ -145,7 +146,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} requested; this build measures exactly one GPU "
-              "(the path is single-GPU by design; see DESIGN.md for how it would shard)",
+              "(the path is single-GPU by design)",
               file=sys.stderr)
         sys.exit(2)
     if args.mode == "batched":
