@@ -36,7 +36,7 @@ hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan
                           const uint32_t *hoi, uint64_t i0, uint64_t i1, unsigned long long *cursor,
                           const uint64_t *E, uint64_t *stk, uint32_t wmax, const KOut &out,
                           const HubQueue &hq, int mode);
-int hub_dfs_occupancy(int nlev, uint32_t wmax);
+int hub_dfs_occupancy(int nlev, uint32_t wmax, bool pair);
 
 template <class T>
 static T *dev_alloc(size_t n)
@@ -419,9 +419,9 @@ static bool hub_donates(const Plan &p, const beep_opts &o, const DevRoots &R)
            !(p.L == 5 && pair_summed(p, o.emit));
 }
 
-static int hub_grid_of(const DevRoots &R, int nlev, int cus)
+static int hub_grid_of(const DevRoots &R, int nlev, bool pair, int cus)
 {
-    return hub_dfs_occupancy(nlev, std::max<uint32_t>(R.wmax, 1)) * cus;
+    return hub_dfs_occupancy(nlev, std::max<uint32_t>(R.wmax, 1), pair) * cus;
 }
 
 ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, int cus)
@@ -435,7 +435,7 @@ ScratchNeeds scratch_needs(const Prep &pp, const Plan &p, const beep_opts &o, in
     const int nlev = p.L > 3 ? p.L - 3 : 1;
     if (R.nhubs) {
         // per wave: nlev stack rows + the row of x_1 (used when they do not fit LDS)
-        n.stk_words = (uint64_t)hub_grid_of(R, nlev, cus) * (kBlock / 64) *
+        n.stk_words = (uint64_t)hub_grid_of(R, nlev, pair_summed(p, o.emit), cus) * (kBlock / 64) *
                       hub_wave_words(nlev, std::max<uint32_t>(R.wmax, 1));
     }
     if (hub_donates(p, o, R)) {
@@ -482,7 +482,7 @@ LaunchCfg make_launch(const Prep &pp, const Plan &p, const beep_opts &o, const S
     c.kp = make_kplan(p, o.emit != BEEP_EMIT_NONE ? 1 : 0, o.build_mode);
     c.kg = KGraph{V.dev->ocol, V.dev->vinfo, V.dev->htab, V.asc};
     c.nlev = p.L > 3 ? p.L - 3 : 1;
-    c.hub_grid = hub_grid_of(R, c.nlev, cus);
+    c.hub_grid = hub_grid_of(R, c.nlev, c.kp.pair != 0, cus);
     // donation ring (hub items with |C_2| > donate_min)
     c.donate = hub_donates(p, o, R) && s.queue && s.slots;
     if (c.donate) {

@@ -162,6 +162,11 @@ __device__ __forceinline__ void mark_stop(const KOut &out) { atomicOr(out.stop, 
 // DFS steps between two deadline checks.
 constexpr uint32_t kClockEvery = 1024;
 
+// Paired last levels of the hub DFS: bit-sliced counters of kPairSlices bits
+// count up to kPairMembers members of S.
+constexpr int kPairSlices = 12;
+constexpr uint32_t kPairMembers = (1u << kPairSlices) - 1u;
+
 // Compute unit the wave runs on, as a slot below 2048: the XCC (HW_REG_XCC_ID
 // bits 2:0) and, within it, shader engine, shader array and CU (HW_REG_HW_ID
 // bits 15:8).  The reference reads %smid for its per-SM times (utils.cuh:66-99).
@@ -1022,7 +1027,7 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 // each a (hub, x_1, slot, [lo, hi)) record appended to the queue with one
 // atomic.  mode 1 (the next launch on the stream) drains the queue: every
 // wave of the grid takes chunks and walks the x_2 in [lo, hi) of its chunk.
-template <bool LDSBUF>
+template <bool LDSBUF, bool PAIR>
 __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                                                          const HubRec *__restrict__ hubs,
                                                          const uint32_t *__restrict__ hub_of_item,
@@ -1157,18 +1162,125 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     };
 
     // Paired last levels (pl.pair), l = L-2 = b: S = C_{l-1} in the stack with
-    // x_1..x_{l-2} fixed; B (level b without its constraints on x_a = x_{l-1})
-    // goes to the next stack row, then C(|S n E[x_b]|, 2) is added for every
-    // x_b in B.
+    // x_1..x_{l-2} fixed, B = level b without its constraints on x_a = x_{l-1}.
+    // The three levels add sum over x_b in B of C(|S n E[x_b]|, 2).
     // Returns true when the deadline passed before it was done (a hub of
     // hundreds of thousands of neighbours makes one call long): the item is
     // left with part of its count.
+    //
+    // Pair plans keep both bits of every adjacent pair (a paired level c
+    // intersects with the row of x_b and is bound only above x_a < x_b), so
+    // |S n E[p]| = c_p = |{s in S : E[s][p]}|, and the sum is
+    //   sum over p in B of C(c_p, 2)
+    // over the rows of S alone.  S's members are listed in the next stack row
+    // (u32); in each round of 64 words a lane keeps the c_p of its word's 64
+    // positions as bit-sliced counters (slice j = bit j of every c_p), adds
+    // the members' words with a ripple carry, and at the round's end takes
+    //   sum_p C(c_p, 2) = sum_j 2^(j-1) (2^j - 1) |sl_j|
+    //                     + sum_{i<j} 2^(i+j) |sl_i & sl_j|.
+    // Every member row is read once, where B can hold candidates.  An S with
+    // more members than the counters or the list hold takes the walk below:
+    // B restricted to the union of the members' rows, then C(|S n E[x_b]|, 2)
+    // for every x_b in it.
     auto pair_count = [&](int l) -> bool {
         uint32_t steps = 0;
         const uint64_t *S = my + (uint64_t)(l - 3) * wmax;
         uint64_t *Bv = my + (uint64_t)(l - 2) * wmax;
         const uint32_t blo = word_lo(l);
         const uint32_t slo = sWlo[wv][l - 1];
+        // B's mask on word w (x_a's constraints left out)
+        auto b_word = [&](uint32_t w) -> uint64_t {
+            uint64_t t = valid_mask(w, d);
+            uint32_t qm = sQM[l] & ~(1u << (l - 1));
+            while (qm && t) {
+                const int q = __builtin_ctz(qm);
+                qm &= qm - 1;
+                if (q == 1 && row1) t &= e1[w];
+                else t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
+            }
+            for (int s = 1; s < l - 1; s++) t &= ~bit_at(xget(s), w);
+            uint32_t sm = sSM[l];
+            while (sm) {
+                const int s = __builtin_ctz(sm);
+                sm &= sm - 1;
+                t &= gt_mask(xget(s), w);
+            }
+            return t;
+        };
+        {
+            uint32_t *list = (uint32_t *)Bv;
+            const uint32_t cap = 2u * wmax < kPairMembers ? 2u * wmax : kPairMembers;
+            uint32_t ns = 0;
+            for (uint32_t rr = slo >> 6; rr < R; rr++) {
+                const uint32_t w = rr * 64 + lane;
+                uint64_t v = (w < W && w >= slo) ? S[w] : 0ull;
+                const uint32_t pc = (uint32_t)__popcll(v);
+                const uint32_t incl = wave_incl_scan(pc);
+                uint32_t at = ns + incl - pc;
+                if (ns + incl <= cap) {
+                    while (v) {
+                        list[at++] = (w << 6) + (uint32_t)__builtin_ctzll(v);
+                        v &= v - 1;
+                    }
+                }
+                ns += __shfl(incl, 63, 64);
+            }
+            if (ns == 0u) return false;
+            if (ns <= cap) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const int nb = 32 - __builtin_clz(ns);  // bits of the largest c_p
+                for (uint32_t rr = blo >> 6; rr < R; rr++) {
+                    if ((++steps & 15u) == 0u && past(dl)) return true;
+                    const uint32_t w = rr * 64 + lane;
+                    const uint64_t bc = (w < W && w >= blo) ? b_word(w) : 0ull;
+                    if (__ballot(bc != 0ull) == 0ull) continue;
+                    uint64_t sl[kPairSlices];
+#pragma unroll
+                    for (int j = 0; j < kPairSlices; j++) sl[j] = 0ull;
+                    auto add = [&](uint64_t c) {
+#pragma unroll
+                        for (int j = 0; j < kPairSlices; j++) {
+                            if (c) {
+                                const uint64_t t = sl[j] & c;
+                                sl[j] ^= c;
+                                c = t;
+                            }
+                        }
+                    };
+                    auto word_of = [&](uint32_t s) -> uint64_t {
+                        return bc ? Eh[(uint64_t)(s - rlo) * W + w] & bc : 0ull;
+                    };
+                    for (uint32_t b0 = 0; b0 < ns; b0 += 64) {
+                        const uint32_t mine = b0 + lane < ns ? list[b0 + lane] : 0u;
+                        const uint32_t nk = ns - b0 < 64u ? ns - b0 : 64u;
+                        uint32_t k = 0;
+                        for (; k + 4 <= nk; k += 4) {  // four member words in flight
+                            const uint64_t r0 = word_of(__builtin_amdgcn_readlane(mine, k));
+                            const uint64_t r1 = word_of(__builtin_amdgcn_readlane(mine, k + 1));
+                            const uint64_t r2 = word_of(__builtin_amdgcn_readlane(mine, k + 2));
+                            const uint64_t r3 = word_of(__builtin_amdgcn_readlane(mine, k + 3));
+                            add(r0);
+                            add(r1);
+                            add(r2);
+                            add(r3);
+                        }
+                        for (; k < nk; k++) add(word_of(__builtin_amdgcn_readlane(mine, k)));
+                    }
+                    uint64_t acc = 0;
+#pragma unroll
+                    for (int j = 1; j < kPairSlices; j++) {
+                        if (j < nb) {
+                            acc += ((1ull << (j - 1)) * ((1ull << j) - 1ull)) * (uint64_t)__popcll(sl[j]);
+#pragma unroll
+                            for (int i = 0; i < j; i++) acc += (uint64_t)__popcll(sl[i] & sl[j]) << (i + j);
+                        }
+                    }
+                    cnt += acc;
+                }
+                return false;
+            }
+        }
         // only an x_b adjacent to some member of S adds anything: B starts as
         // the union of their rows (from word blo on)
         for (uint32_t rr = blo >> 6; rr < R; rr++) {
@@ -1203,25 +1315,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         for (uint32_t rr = 0; rr < R; rr++) {
             const uint32_t w = rr * 64 + lane;
             if (w >= W) break;
-            uint64_t t = 0;
-            if (w >= blo) {
-                t = valid_mask(w, d) & Bv[w];
-                uint32_t qm = sQM[l] & ~(1u << (l - 1));
-                while (qm) {
-                    const int q = __builtin_ctz(qm);
-                    qm &= qm - 1;
-                    if (q == 1 && row1) t &= e1[w];
-                    else t &= Eh[(uint64_t)(xget(q) - rlo) * W + w];
-                }
-                for (int s = 1; s < l - 1; s++) t &= ~bit_at(xget(s), w);
-                uint32_t sm = sSM[l];
-                while (sm) {
-                    const int s = __builtin_ctz(sm);
-                    sm &= sm - 1;
-                    t &= gt_mask(xget(s), w);
-                }
-            }
-            Bv[w] = t;
+            Bv[w] = (w >= blo && Bv[w]) ? Bv[w] & b_word(w) : 0ull;
         }
         __builtin_amdgcn_wave_barrier();
         uint32_t pos = blo << 6;
@@ -1274,7 +1368,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 pos = xget(l - 1) + 1u;
                 continue;
             }
-            if (pl.pair && l == L - 2) {  // levels L-3..L-1 at once (never split by donation at l = 3)
+            if (PAIR && l == L - 2) {  // levels L-3..L-1 at once (never split by donation at l = 3)
                 if (pair_count(l)) break;  // the next claim records the stop
                 if (l == 3) break;
                 --l;
@@ -1634,10 +1728,21 @@ constexpr size_t kHubLdsMax = 96 * 1024;
 
 bool hub_dfs_in_lds(int nlev, uint32_t wmax) { return hub_lds_bytes(nlev, wmax) <= kHubLdsMax; }
 
-static LdsLaunchCache &hub_dfs_lds_cache()
+// one cache per instantiation: [in LDS][pair]
+static LdsLaunchCache &hub_dfs_cache(bool in_lds, bool pair)
 {
-    static LdsLaunchCache c;
-    return c;
+    static LdsLaunchCache c[2][2];
+    return c[in_lds][pair];
+}
+
+// The instantiation a plan runs: the stack in LDS or HBM, and the paired last
+// levels compiled in only for the plans that have them (their counters cost
+// registers, hence occupancy, that the other plans keep).
+static const void *hub_dfs_fn(int nlev, uint32_t wmax, bool pair)
+{
+    if (hub_dfs_in_lds(nlev, wmax))
+        return pair ? (const void *)hub_dfs_kernel<true, true> : (const void *)hub_dfs_kernel<true, false>;
+    return pair ? (const void *)hub_dfs_kernel<false, true> : (const void *)hub_dfs_kernel<false, false>;
 }
 
 hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan &pl, const HubRec *hubs,
@@ -1646,24 +1751,21 @@ hipError_t launch_hub_dfs(int grid, hipStream_t st, const KGraph &g, const KPlan
                           const HubQueue &hq, int mode)
 {
     const int nlev = pl.L > 3 ? pl.L - 3 : 1;
-    if (hub_dfs_in_lds(nlev, wmax)) {
-        const size_t lds = hub_lds_bytes(nlev, wmax);
-        (void)hub_dfs_lds_cache().get((const void *)hub_dfs_kernel<true>, lds);
-        hipLaunchKernelGGL(hub_dfs_kernel<true>, dim3(grid), dim3(kBlock), lds, st, g, pl, hubs, hoi, i0, i1,
-                           cursor, E, stk, wmax, out, hq, mode);
-    } else {
-        hipLaunchKernelGGL(hub_dfs_kernel<false>, dim3(grid), dim3(kBlock), 0, st, g, pl, hubs, hoi, i0, i1,
-                           cursor, E, stk, wmax, out, hq, mode);
-    }
-    return hipGetLastError();
+    const bool in_lds = hub_dfs_in_lds(nlev, wmax);
+    const size_t lds = in_lds ? hub_lds_bytes(nlev, wmax) : 0;
+    const void *fn = hub_dfs_fn(nlev, wmax, pl.pair != 0);
+    if (in_lds) (void)hub_dfs_cache(true, pl.pair != 0).get(fn, lds);
+    void *args[] = {(void *)&g,  (void *)&pl,     (void *)&hubs, (void *)&hoi, (void *)&i0,
+                    (void *)&i1, (void *)&cursor, (void *)&E,    (void *)&stk, (void *)&wmax,
+                    (void *)&out, (void *)&hq,    (void *)&mode};
+    return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, lds, st);
 }
 
-int hub_dfs_occupancy(int nlev, uint32_t wmax)
+int hub_dfs_occupancy(int nlev, uint32_t wmax, bool pair)
 {
-    if (hub_dfs_in_lds(nlev, wmax))
-        return hub_dfs_lds_cache().get((const void *)hub_dfs_kernel<true>, hub_lds_bytes(nlev, wmax));
-    static LdsLaunchCache no_lds;
-    return no_lds.get((const void *)hub_dfs_kernel<false>, 0);
+    const void *fn = hub_dfs_fn(nlev, wmax, pair);
+    const bool in_lds = hub_dfs_in_lds(nlev, wmax);
+    return hub_dfs_cache(in_lds, pair).get(fn, in_lds ? hub_lds_bytes(nlev, wmax) : 0);
 }
 
 }  // namespace beep
