@@ -410,13 +410,12 @@ static bool pair_summed(const Plan &p, int emit)
 }
 
 // Hub items donate level-2 subtrees; not when everything below x_1 is one
-// summed level (L = 4 with a summed tail) or one summed pair of levels (L = 5
-// with paired levels): an item then costs one or |B| passes over its d/64
-// words whatever |C_2| is.
+// summed level (L = 4 with a summed tail): an item then costs one pass over
+// its d/64 words whatever |C_2| is.  Items of 5-vertex plans with paired
+// levels donate ranges of the paired pass's words instead.
 static bool hub_donates(const Plan &p, const beep_opts &o, const DevRoots &R)
 {
-    return o.donate && R.nhubs && p.L >= 4 && !(p.L == 4 && tail_summed(p, o.emit)) &&
-           !(p.L == 5 && pair_summed(p, o.emit));
+    return o.donate && R.nhubs && p.L >= 4 && !(p.L == 4 && tail_summed(p, o.emit));
 }
 
 static int hub_grid_of(const DevRoots &R, int nlev, bool pair, int cus)

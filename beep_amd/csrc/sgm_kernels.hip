@@ -163,9 +163,15 @@ __device__ __forceinline__ void mark_stop(const KOut &out) { atomicOr(out.stop, 
 constexpr uint32_t kClockEvery = 1024;
 
 // Paired last levels of the hub DFS: bit-sliced counters of kPairSlices bits
-// count up to kPairMembers members of S.
-constexpr int kPairSlices = 12;
+// count up to kPairMembers members of S.  The instantiation with them is held
+// to 3 waves per SIMD (168 VGPRs).
+constexpr int kPairSlices = 16;
 constexpr uint32_t kPairMembers = (1u << kPairSlices) - 1u;
+// A 5-vertex pair plan's hub item of more than kPairSplitWork member-rounds
+// (|C_2| x rounds of 64 words of B) is donated in parts of about
+// kPairChunkWork member-rounds, whole rounds each.
+constexpr uint64_t kPairSplitWork = 1u << 13;
+constexpr uint32_t kPairChunkWork = 1u << 12;
 
 // Compute unit the wave runs on, as a slot below 2048: the XCC (HW_REG_XCC_ID
 // bits 2:0) and, within it, shader engine, shader array and CU (HW_REG_HW_ID
@@ -1028,7 +1034,7 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 // atomic.  mode 1 (the next launch on the stream) drains the queue: every
 // wave of the grid takes chunks and walks the x_2 in [lo, hi) of its chunk.
 template <bool LDSBUF, bool PAIR>
-__global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
+__global__ __launch_bounds__(kBlock, PAIR ? 3 : 1) void hub_dfs_kernel(KGraph g, KPlan pl,
                                                          const HubRec *__restrict__ hubs,
                                                          const uint32_t *__restrict__ hub_of_item,
                                                          uint64_t i0, uint64_t i1,
@@ -1162,32 +1168,38 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
     };
 
     // Paired last levels (pl.pair), l = L-2 = b: S = C_{l-1} in the stack with
-    // x_1..x_{l-2} fixed, B = level b without its constraints on x_a = x_{l-1}.
-    // The three levels add sum over x_b in B of C(|S n E[x_b]|, 2).
-    // Returns true when the deadline passed before it was done (a hub of
-    // hundreds of thousands of neighbours makes one call long): the item is
-    // left with part of its count.
+    // x_1..x_{l-2} fixed, B = level b without its constraints on x_a = x_{l-1},
+    // cut to the positions [plo, phi) (a donated part of an item; words at
+    // multiples of 64).  The three levels add sum over x_b in B of
+    // C(|S n E[x_b]|, 2).  Returns true when the deadline passed before it was
+    // done (a hub of hundreds of thousands of neighbours makes one call long):
+    // the item is left with part of its count.
     //
     // Pair plans keep both bits of every adjacent pair (a paired level c
     // intersects with the row of x_b and is bound only above x_a < x_b), so
     // |S n E[p]| = c_p = |{s in S : E[s][p]}|, and the sum is
     //   sum over p in B of C(c_p, 2)
-    // over the rows of S alone.  S's members are listed in the next stack row
-    // (u32); in each round of 64 words a lane keeps the c_p of its word's 64
-    // positions as bit-sliced counters (slice j = bit j of every c_p), adds
-    // the members' words with a ripple carry, and at the round's end takes
+    // over the rows of S alone.  In each round of 64 words a lane keeps the
+    // c_p of its word's 64 positions as bit-sliced counters (slice j = bit j
+    // of every c_p), adds the members' words with a ripple carry, and at the
+    // round's end takes
     //   sum_p C(c_p, 2) = sum_j 2^(j-1) (2^j - 1) |sl_j|
     //                     + sum_{i<j} 2^(i+j) |sl_i & sl_j|.
-    // Every member row is read once, where B can hold candidates.  An S with
-    // more members than the counters or the list hold takes the walk below:
-    // B restricted to the union of the members' rows, then C(|S n E[x_b]|, 2)
-    // for every x_b in it.
-    auto pair_count = [&](int l) -> bool {
+    // Every member row is read once per round where B can hold candidates.
+    // The members come from a list in the next stack row (u32, two rows in
+    // flight) when it holds them, else straight from S's words.  An S with more
+    // members than the counters hold takes the walk at the end: B restricted
+    // to the union of the members' rows, then C(|S n E[x_b]|, 2) for every x_b
+    // in it.
+    auto pair_count = [&](int l, uint32_t plo, uint32_t phi) -> bool {
         uint32_t steps = 0;
         const uint64_t *S = my + (uint64_t)(l - 3) * wmax;
         uint64_t *Bv = my + (uint64_t)(l - 2) * wmax;
-        const uint32_t blo = word_lo(l);
+        uint32_t blo = word_lo(l);
+        if ((plo >> 6) > blo) blo = plo >> 6;
+        const uint32_t bhi = phi >= (W << 6) ? W : (phi >> 6);  // B's words: [blo, bhi)
         const uint32_t slo = sWlo[wv][l - 1];
+        if (blo >= bhi) return false;
         // B's mask on word w (x_a's constraints left out)
         auto b_word = [&](uint32_t w) -> uint64_t {
             uint64_t t = valid_mask(w, d);
@@ -1226,14 +1238,17 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 ns += __shfl(incl, 63, 64);
             }
             if (ns == 0u) return false;
-            if (ns <= cap) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            if (ns <= kPairMembers) {
+                const bool listed = ns <= cap;
+                if (listed) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
                 const int nb = 32 - __builtin_clz(ns);  // bits of the largest c_p
-                for (uint32_t rr = blo >> 6; rr < R; rr++) {
+                for (uint32_t rr = blo >> 6; rr * 64 < bhi; rr++) {
                     if ((++steps & 15u) == 0u && past(dl)) return true;
                     const uint32_t w = rr * 64 + lane;
-                    const uint64_t bc = (w < W && w >= blo) ? b_word(w) : 0ull;
+                    const uint64_t bc = (w < bhi && w >= blo) ? b_word(w) : 0ull;
                     if (__ballot(bc != 0ull) == 0ull) continue;
                     uint64_t sl[kPairSlices];
 #pragma unroll
@@ -1251,21 +1266,32 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                     auto word_of = [&](uint32_t s) -> uint64_t {
                         return bc ? Eh[(uint64_t)(s - rlo) * W + w] & bc : 0ull;
                     };
-                    for (uint32_t b0 = 0; b0 < ns; b0 += 64) {
-                        const uint32_t mine = b0 + lane < ns ? list[b0 + lane] : 0u;
-                        const uint32_t nk = ns - b0 < 64u ? ns - b0 : 64u;
-                        uint32_t k = 0;
-                        for (; k + 4 <= nk; k += 4) {  // four member words in flight
-                            const uint64_t r0 = word_of(__builtin_amdgcn_readlane(mine, k));
-                            const uint64_t r1 = word_of(__builtin_amdgcn_readlane(mine, k + 1));
-                            const uint64_t r2 = word_of(__builtin_amdgcn_readlane(mine, k + 2));
-                            const uint64_t r3 = word_of(__builtin_amdgcn_readlane(mine, k + 3));
-                            add(r0);
-                            add(r1);
-                            add(r2);
-                            add(r3);
+                    if (listed) {
+                        for (uint32_t b0 = 0; b0 < ns; b0 += 64) {
+                            const uint32_t mine = b0 + lane < ns ? list[b0 + lane] : 0u;
+                            const uint32_t nk = ns - b0 < 64u ? ns - b0 : 64u;
+                            uint32_t k = 0;
+                            for (; k + 2 <= nk; k += 2) {  // two member words in flight
+                                const uint64_t r0 = word_of(__builtin_amdgcn_readlane(mine, k));
+                                const uint64_t r1 = word_of(__builtin_amdgcn_readlane(mine, k + 1));
+                                add(r0);
+                                add(r1);
+                            }
+                            for (; k < nk; k++) add(word_of(__builtin_amdgcn_readlane(mine, k)));
                         }
-                        for (; k < nk; k++) add(word_of(__builtin_amdgcn_readlane(mine, k)));
+                    } else {
+                        for (uint32_t sr = slo >> 6; sr < R; sr++) {  // S's words, 64 at a time
+                            const uint32_t sw = sr * 64 + lane;
+                            uint64_t v = (sw < W && sw >= slo) ? S[sw] : 0ull;
+                            for (;;) {
+                                const uint64_t bal = __ballot(v != 0ull);
+                                if (bal == 0ull) break;
+                                const int f = __builtin_ctzll(bal);
+                                const uint64_t vf = __shfl(v, f, 64);
+                                if (lane == f) v &= v - 1ull;
+                                add(word_of(((sr * 64 + (uint32_t)f) << 6) + (uint32_t)__builtin_ctzll(vf)));
+                            }
+                        }
                     }
                     uint64_t acc = 0;
 #pragma unroll
@@ -1282,10 +1308,10 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             }
         }
         // only an x_b adjacent to some member of S adds anything: B starts as
-        // the union of their rows (from word blo on)
+        // the union of their rows (words [blo, bhi))
         for (uint32_t rr = blo >> 6; rr < R; rr++) {
             const uint32_t w = rr * 64 + lane;
-            if (w < W && w >= blo) Bv[w] = 0ull;
+            if (w < bhi && w >= blo) Bv[w] = 0ull;
         }
         for (uint32_t pos = slo << 6;;) {
             uint32_t xs = 0xffffffffu;
@@ -1307,7 +1333,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             const uint64_t *Es = Eh + (uint64_t)(xs - rlo) * W;  // x_a's row: kept, as the walk reads it
             for (uint32_t rr = blo >> 6; rr < R; rr++) {
                 const uint32_t w = rr * 64 + lane;
-                if (w < W && w >= blo) Bv[w] |= Es[w];
+                if (w < bhi && w >= blo) Bv[w] |= Es[w];
             }
             pos = xs + 1u;
         }
@@ -1315,7 +1341,7 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
         for (uint32_t rr = 0; rr < R; rr++) {
             const uint32_t w = rr * 64 + lane;
             if (w >= W) break;
-            Bv[w] = (w >= blo && Bv[w]) ? Bv[w] & b_word(w) : 0ull;
+            Bv[w] = (w >= blo && w < bhi && Bv[w]) ? Bv[w] & b_word(w) : 0ull;
         }
         __builtin_amdgcn_wave_barrier();
         uint32_t pos = blo << 6;
@@ -1368,8 +1394,9 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
                 pos = xget(l - 1) + 1u;
                 continue;
             }
-            if (PAIR && l == L - 2) {  // levels L-3..L-1 at once (never split by donation at l = 3)
-                if (pair_count(l)) break;  // the next claim records the stop
+            if (PAIR && l == L - 2) {  // levels L-3..L-1 at once; at l = 3 [pos0, hi2) cuts B
+                const bool top = l == 3;
+                if (pair_count(l, top ? pos0 : 0u, top ? hi2 : 0xffffffffu)) break;  // the next claim records the stop
                 if (l == 3) break;
                 --l;
                 pos = xget(l - 1) + 1u;
@@ -1514,6 +1541,48 @@ __global__ __launch_bounds__(kBlock) void hub_dfs_kernel(KGraph g, KPlan pl,
             }
         }
         c2 = (uint32_t)wave_sum_u64(c2);
+        if (PAIR && L == 5) {
+            // The item is one paired pass over B's words with S = C_2; a long one
+            // donates ranges of B's rounds (the sum splits over B's positions;
+            // B's bounds depend on x_1 only).
+            const uint32_t r0 = word_lo(3) >> 6;
+            const uint32_t nr = R > r0 ? R - r0 : 0u;
+            uint32_t rpc = c2 ? kPairChunkWork / c2 : nr;
+            if (rpc < 1u) rpc = 1u;
+            const uint32_t nc = (nr + rpc - 1u) / rpc;
+            if (hq.q && nc >= 2u && (uint64_t)c2 * nr > kPairSplitWork) {
+                unsigned long long sb = 0;
+                if (lane == 0) sb = atomicAdd(hq.slot_tail, (unsigned long long)W);
+                sb = __shfl(sb, 0, 64);
+                if (sb + W <= hq.slot_cap) {
+                    unsigned long long qb = 0;
+                    if (lane == 0) qb = atomicAdd(hq.tail, (unsigned long long)nc);
+                    qb = __shfl(qb, 0, 64);
+                    if (qb + nc <= hq.cap) {
+                        if (lane == 0) atomicAdd(hq.donated, (unsigned long long)nc);
+                        for (uint32_t rr = 0; rr < R; rr++) {
+                            const uint32_t w = rr * 64 + lane;
+                            if (w < W) hq.slots[sb + w] = my[w];
+                        }
+                        for (uint32_t c = lane; c < nc; c += 64) {
+                            HubChunk *rec = hq.q + qb + c;
+                            rec->hub = hidx;
+                            rec->x1 = jl;
+                            rec->slot = sb;
+                            rec->lo = c ? (r0 + c * rpc) << 12 : 0u;
+                            rec->hi = c + 1u == nc ? (W << 6) : (r0 + (c + 1u) * rpc) << 12;
+                        }
+                        continue;
+                    }
+                    for (uint64_t c = qb + lane; c < qb + nc && c < hq.cap; c += 64) {
+                        hq.q[c].lo = 0;
+                        hq.q[c].hi = 0;
+                    }
+                }
+            }
+            dfs3(0, 0xffffffffu);
+            continue;
+        }
         if (hq.q && c2 > hq.split_min) {
             // donate: publish C_2, then cut it into chunks of hq.chunk candidates
             unsigned long long sb = 0;
