@@ -163,9 +163,10 @@ __device__ __forceinline__ void mark_stop(const KOut &out) { atomicOr(out.stop, 
 constexpr uint32_t kClockEvery = 1024;
 
 // Paired last levels of the hub DFS: bit-sliced counters of kPairSlices bits
-// count up to kPairMembers members of S.  The instantiation with them is held
-// to 3 waves per SIMD (168 VGPRs).
-constexpr int kPairSlices = 16;
+// count up to kPairMembers members of S (more than any hub row holds below the
+// root window's 100 x cutoff on graphs of up to 2^26 vertices).  The
+// instantiation with them is held to 3 waves per SIMD (168 VGPRs, a few spilled).
+constexpr int kPairSlices = 20;
 constexpr uint32_t kPairMembers = (1u << kPairSlices) - 1u;
 // A 5-vertex pair plan's hub item of more than kPairSplitWork member-rounds
 // (|C_2| x rounds of 64 words of B) is donated in parts of about

@@ -14,7 +14,7 @@ The kernels themselves are pinned by the count-only GPU tests (W5 and the
 other pair plans) against the oracle."""
 import random
 
-SLICES = 12
+SLICES = 20
 
 
 def popc(x):
