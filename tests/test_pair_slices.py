@@ -117,3 +117,40 @@ def test_asymmetric_bitmap_differs():
                 assert sliced_pair_sum(E, S, B, W) != brute(E, S, B, 90)
                 return
     raise AssertionError("no pair to plant")
+
+
+def donated_word_ranges(W, r0, c2, word_lo, chunk_work=1 << 12):
+    """The hub kernel's cut of a 5-vertex pair item into donated parts (whole
+    rounds of 64 words, about chunk_work member-rounds each; the first part
+    starts at position 0) and the word range [blo, bhi) each part's paired
+    pass then covers, with B's own first word word_lo."""
+    R = (W + 63) // 64
+    nr = R - r0 if R > r0 else 0
+    rpc = max(1, chunk_work // c2 if c2 else nr)
+    nc = (nr + rpc - 1) // rpc
+    out = []
+    for c in range(nc):
+        lo = (r0 + c * rpc) << 12 if c else 0
+        hi = (W << 6) if c + 1 == nc else (r0 + (c + 1) * rpc) << 12
+        blo = max(word_lo, lo >> 6)
+        bhi = W if hi >= (W << 6) else hi >> 6
+        out.append((blo, bhi))
+    return out
+
+
+def test_donated_parts_cover_b_once():
+    rng = random.Random(3)
+    for _ in range(2000):
+        W = rng.randint(65, 20000)
+        R = (W + 63) // 64
+        word_lo = rng.randint(0, W - 1)
+        r0 = word_lo >> 6
+        c2 = rng.choice([1, 2, 7, 63, 64, 65, 1000, 4096, 5000, 70000])
+        parts = donated_word_ranges(W, r0, c2, word_lo)
+        covered = []
+        for blo, bhi in parts:
+            covered.extend(range(blo, max(blo, bhi)))
+        assert covered == list(range(word_lo, W)), (W, word_lo, c2)
+        # parts after the first start on a round (64 words)
+        assert all(blo % 64 == 0 for blo, _ in parts[1:]), (W, word_lo, c2)
+        assert len(parts) <= R - r0
