@@ -198,8 +198,11 @@ typedef struct {
     uint64_t match_cap;     /* BEEP_EMIT_BUFFER: tuples to keep */
     uint32_t *match_buf;    /* host buffer of match_cap * (k - lunmat) u32, or NULL */
     int lanes;              /* 0 = automatic, else lanes per DFS group (1..64) */
-    int donate;             /* 0 = off; 1 = donate level-2 subtrees of hub roots */
-    uint32_t donate_min;    /* donation threshold: min |C_2| of a donated subtree */
+    int donate;             /* 0 = off; 1 = donate level-2 subtrees of hub roots
+                               (count-only 5-vertex plans with paired last levels,
+                               such as W5: ranges of the paired pass instead) */
+    uint32_t donate_min;    /* donation threshold: min |C_2| of a donated subtree
+                               (not used by the paired-pass ranges) */
     int timing;             /* 1 = record HIP events around the enumeration */
     double deadline_ms;     /* > 0: stop the enumeration this long after it starts on
                                the device; the result is then partial and marked stopped */
