@@ -1034,6 +1034,9 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k)
 // each a (hub, x_1, slot, [lo, hi)) record appended to the queue with one
 // atomic.  mode 1 (the next launch on the stream) drains the queue: every
 // wave of the grid takes chunks and walks the x_2 in [lo, hi) of its chunk.
+// PAIR compiles in the paired last levels; a 5-vertex pair plan's item is one
+// paired pass, so its records hold ranges [lo, hi) of the pass's positions
+// of B (whole rounds of 64 words) instead of x_2 ranges.
 template <bool LDSBUF, bool PAIR>
 __global__ __launch_bounds__(kBlock, PAIR ? 3 : 1) void hub_dfs_kernel(KGraph g, KPlan pl,
                                                          const HubRec *__restrict__ hubs,
